@@ -1,0 +1,213 @@
+"""Throughput of the gfx950 PatchStereo engine on one MI355X.
+
+Workload (the reference hot path's single-GPU configuration): seeded fBm terrain (200 x 200 m,
++-15 m relief, random albedo) seen by 7 pinhole cameras 30 m up at 45 deg oblique, f = 500 px,
+1920 x 1080 gray with Gaussian noise sigma = 0.01, quantised to u8. Depth maps for all 7 views,
+each view using the other 6 as sources (1 reference + 6 sources, as the geometric pass needs the
+sources' depth), window radius 5 step 2, depth range 15-120 m, FP32:
+7 photometric iterations from random initialisation, one geometric-consistency round of 2
+iterations, read-out filter and count threshold (crates/dense/src/densify.rs:350-454 single-level
+schedule), all on the device (c3d_pm_densify).
+
+A step = that whole job. Images are uploaded before the timed region; states stay in HBM.
+value = sum of final W*H over the 7 depth maps / step time (Mpx/s).
+"""
+from __future__ import annotations
+
+import argparse
+import ctypes as C
+import json
+import os
+import platform
+import sys
+import time
+
+ROOT = os.path.dirname(os.path.abspath(__file__))
+sys.path.insert(0, ROOT)
+
+import numpy as np  # noqa: E402
+
+from cumulus3d_amd import abi, pm, scenes  # noqa: E402
+
+HBM_PEAK_GBS = 8000.0
+FP32_PEAK_TFLOPS = 157.3
+FLOP_PER_SAMPLE = 28  # projective transform 9, reciprocal 1, bilinear 7, scale 1, accumulations 6 (rounded up)
+
+
+def log(msg):
+    print(f"[bench] {msg}", file=sys.stderr, flush=True)
+
+
+def device_sync():
+    try:
+        hip = C.CDLL("libamdhip64.so")
+        hip.hipDeviceSynchronize()
+    except OSError:
+        pass
+
+
+def make_workload(width, height, nviews, noise, seed):
+    rig = scenes.oblique_rig(nviews, baseline=6.0)
+    sc = scenes.make_terrain_scene(rig, width, height, focal=500.0, seed=seed, noise_sigma=noise, noise_seed=seed + 1)
+    src = [[j for j in range(nviews) if j != v] for v in range(nviews)]
+    pmp, fl = abi.fast_profile()
+    inp = scenes.kernel_input(sc, src, [(15.0, 120.0)] * nviews, pmp, fl)
+    return sc, inp
+
+
+def cpu_baseline(inp, rows, threads, photometric_iters):
+    """The CPU oracle (C, OpenMP over rows) on a bounded sample: a band of `rows` rows of view 0,
+    random-init photometric run (evaluation of the band and its 24-row reach, then the iterations
+    on the band only). Returns the rate in pixel-iterations per second (eval counted as one)."""
+    import oracle
+
+    o = oracle.Oracle(inp, threads=threads)
+    h, w = inp.level_shape(0, 0)
+    st = abi.ViewState(w, h)
+    p = abi.run_params(0, False, True, photometric_iters, 1, False)
+    r0 = h // 2 - rows // 2
+    t = time.perf_counter()
+    o.run_rows(0, st, p, r0, r0 + rows)
+    dt = time.perf_counter() - t
+    px_iters = w * rows * photometric_iters + w * min(h, rows + 48)
+    return px_iters / dt, dt, f"view 0, rows {r0}..{r0 + rows} of {w}x{h}, random init + {photometric_iters} photometric iterations"
+
+
+def main():
+    ap = argparse.ArgumentParser()
+    ap.add_argument("--gpus", type=int, default=1)
+    ap.add_argument("--steps", type=int, default=3)
+    ap.add_argument("--warmup", type=int, default=1)
+    ap.add_argument("--width", type=int, default=1920)
+    ap.add_argument("--height", type=int, default=1080)
+    ap.add_argument("--views", type=int, default=7)
+    ap.add_argument("--photometric-iters", type=int, default=7)
+    ap.add_argument("--geometric-rounds", type=int, default=1)
+    ap.add_argument("--geometric-iters", type=int, default=2)
+    ap.add_argument("--cpu-rows", type=int, default=320)
+    ap.add_argument("--no-cpu", action="store_true")
+    ap.add_argument("--repro", action="store_true", help="use the reproducible-reciprocal build")
+    args = ap.parse_args()
+    if args.gpus != 1:
+        print(f"bench.py measures one GPU only (every box here has one); --gpus {args.gpus} is not supported", file=sys.stderr)
+        sys.exit(2)
+    if not pm.is_available(args.repro):
+        print("no gfx950 device: the engine has no CPU path", file=sys.stderr)
+        sys.exit(1)
+
+    t0 = time.perf_counter()
+    sc, inp = make_workload(args.width, args.height, args.views, 0.01, seed=1)
+    log(f"scene {args.views} x {args.width}x{args.height} rendered in {time.perf_counter() - t0:.1f} s")
+    views = list(range(args.views))
+    npix = args.width * args.height
+    be = pm.HipPatchMatch(0, repro=args.repro)
+    sess = be.begin(inp)
+    log(f"session open on {be.name()}")
+    for i in range(args.warmup):
+        sess.densify(views, args.photometric_iters, args.geometric_rounds, args.geometric_iters, download=False)
+        log(f"warmup {i + 1}/{args.warmup} done")
+    device_sync()
+    times, tms = [], []
+    t_all = time.perf_counter()
+    for i in range(args.steps):
+        ts = time.perf_counter()
+        _, _, _, tm = sess.densify(views, args.photometric_iters, args.geometric_rounds, args.geometric_iters, download=False)
+        times.append(time.perf_counter() - ts)
+        tms.append(tm)
+        log(f"step {i + 1}/{args.steps}: {times[-1] * 1e3:.1f} ms (device {tm.total_ms:.1f} ms)")
+    device_sync()
+    wall = time.perf_counter() - t_all
+    ms_per_step = wall / args.steps * 1e3
+    job_mpx = args.views * npix / 1e6
+    value = job_mpx / (ms_per_step / 1e3)
+
+    # dominant kernel: the red-black half step. Algorithmic work per pixel-iteration (one pixel
+    # updated once by a red + black pair): S = 15 nsrc WN^2 bilinear samples (upper bound: 9
+    # propagated + 6 refinement hypotheses over every source), 28 FLOP each; compulsory HBM bytes
+    # 58 + 2 (1 + nsrc), plus 8 nsrc for geometric iterations. One launch updates npix/2 pixels
+    # of every view.
+    nsrc = args.views - 1
+    wn = (2 * inp.pm.window_radius) // inp.pm.window_step + 1
+    n_photo = 2 * args.photometric_iters
+    n_geo = 2 * args.geometric_iters * args.geometric_rounds
+    px_per_launch = args.views * npix / 2
+    bytes_photo = 58 + 2 * (1 + nsrc)
+    bytes_geo = bytes_photo + 8 * nsrc
+    avg_bytes = (n_photo * bytes_photo + n_geo * bytes_geo) / (n_photo + n_geo) * px_per_launch
+    flop = FLOP_PER_SAMPLE * 15 * nsrc * wn * wn * px_per_launch
+    half_ms = sum(t.half_kernel_ms for t in tms) / sum(t.half_launches for t in tms)
+    achieved_gbs = avg_bytes / (half_ms / 1e3) / 1e9
+    achieved_tflops = flop / (half_ms / 1e3) / 1e12
+    line = {
+        "metric": "depth-map Mpx/s at <=1% rel. depth error vs plain PatchStereo; HBM GB/s",
+        "value": round(value, 3),
+        "unit": "Mpx/s",
+        "n_gpus": 1,
+        "steps": args.steps,
+        "warmup": args.warmup,
+        "ms_per_step": round(ms_per_step, 2),
+        "higher_is_better": True,
+        "scaling": "weak",
+        "vs_baseline": None,
+        "dtype": "f32",
+        "data": "synthetic (seeded fBm terrain, rendered on the host)",
+        "config": {
+            "workload": f"plain PatchStereo, {args.views} views {args.width}x{args.height} u8, 1 ref + {nsrc} src each, "
+                        f"window 5 step 2, {args.photometric_iters} photometric + {args.geometric_rounds}x{args.geometric_iters} "
+                        f"geometric iterations, read-out filter, depth 15-120 m",
+            "views": args.views,
+            "width": args.width,
+            "height": args.height,
+            "sources_per_view": nsrc,
+            "photometric_iters": args.photometric_iters,
+            "geometric_rounds": args.geometric_rounds,
+            "geometric_iters": args.geometric_iters,
+            "library": "repro" if args.repro else "production",
+        },
+        "roofline": {
+            "bound": "hbm",
+            "kernel": "k_half<5,2>",
+            "achieved": round(achieved_gbs, 2),
+            "peak": HBM_PEAK_GBS,
+            "unit": "GB/s",
+            "frac": round(achieved_gbs / HBM_PEAK_GBS, 5),
+            "traffic": None,
+            "avg_launch_ms": round(half_ms, 3),
+            "algorithmic_bytes_per_launch": int(avg_bytes),
+        },
+        "roofline_valu": {
+            "bound": "valu-fp32",
+            "kernel": "k_half<5,2>",
+            "achieved": round(achieved_tflops, 2),
+            "peak": FP32_PEAK_TFLOPS,
+            "unit": "TFLOP/s",
+            "frac": round(achieved_tflops / FP32_PEAK_TFLOPS, 4),
+            "algorithmic_flop_per_launch": int(flop),
+        },
+        "stage_ms": {
+            "photometric": round(sum(t.photometric_ms for t in tms) / len(tms), 2),
+            "geometric": round(sum(t.geometric_ms for t in tms) / len(tms), 2),
+            "filter": round(sum(t.filter_ms for t in tms) / len(tms), 2),
+        },
+    }
+    sess.close()
+    if not args.no_cpu:
+        threads = min(os.cpu_count() or 1, 16)
+        rate, dt, sample = cpu_baseline(inp, args.cpu_rows, threads, args.photometric_iters)
+        job_px_iters = args.views * npix * (args.photometric_iters + 1 + args.geometric_rounds * (args.geometric_iters + 1) + 1)
+        cpu_mpxs = job_mpx / (job_px_iters / rate)
+        line["cpu_baseline"] = {
+            "value": round(cpu_mpxs, 5),
+            "unit": "Mpx/s",
+            "cores": threads,
+            "kind": "port",
+            "sample": f"{sample}: {dt:.1f} s, {rate:.0f} px-iter/s, scaled to the job's "
+                      f"{job_px_iters / 1e6:.1f} M px-iters (the reference has no CPU PatchStereo; this is the C oracle, "
+                      f"{platform.processor() or platform.machine()})",
+        }
+        log(f"cpu baseline: {rate:.0f} px-iter/s on {threads} threads ({dt:.1f} s)")
+    print(json.dumps(line), flush=True)
+
+
+if __name__ == "__main__":
+    main()

@@ -1,0 +1,620 @@
+// pm_kernels.hip — gfx950 PatchStereo kernels (random init + evaluation, red-black half step,
+// read-out filter, upsampling, median filter, cost probe) and their launchers.
+//
+// The kernels follow the rules of crates/dense/src/kernel.rs:36-74 as implemented by the reference's
+// crates/cuda/src/kernels/patchmatch.cu (cited per kernel). Work decomposition (MI355X-specific):
+//   * wave64 = 64 active pixels of one row (red-black: every other pixel), 4 rows per 256-thread
+//     workgroup; a workgroup stages its reference patch rows (128 + 2R px wide) in LDS once;
+//   * all views of a batch share one flattened 1-D grid of tiles (no empty blocks for smaller
+//     views), remapped so that consecutive tiles run on the same XCD and share its L2;
+//   * the per-thread view weights of the refinement live in LDS, everything else in VGPRs; the
+//     hypothesis and source loops stay rolled so that one inlined NCC body serves all hypotheses.
+#include "pm_device.hpp"
+#include "pm_kernels.hpp"
+
+namespace c3d {
+
+constexpr int HALF_ROWS = 4;     // rows of active pixels per workgroup
+constexpr int HALF_SPAN = 128;   // pixel span of one row of 64 active pixels
+constexpr int PIX_TW = 64;       // eval/filter tiles: 64 x 4 pixels
+constexpr int PIX_TH = 4;
+
+// region offsets for the up direction (patchmatch.cu:424-436)
+__constant__ int V_OFF[7][2] = {{0, -1}, {-1, -2}, {1, -2}, {-2, -3}, {2, -3}, {-3, -4}, {3, -4}};
+
+__device__ __forceinline__ void rot_dir(int dir, int dx, int dy, int& ox, int& oy) {
+    ox = dx;
+    oy = dy;
+    for (int k = 0; k < dir; ++k) {
+        int t = ox;
+        ox = -oy;
+        oy = t;
+    }
+}
+
+template <int N>
+__device__ __forceinline__ float4 pick4(const float4 (&a)[N], int i) {
+    float4 r = a[0];
+#pragma unroll
+    for (int q = 1; q < N; ++q)
+        if (q == i) r = a[q];
+    return r;
+}
+
+// Weighted sum of one hypothesis' costs over the sources with positive weight (refinement and
+// coarse-prior candidates, patchmatch.cu:608-652).
+template <int WR, int WSTEP>
+__device__ __forceinline__ float candidate_cost(const KParams& P, const RefWin<WR, WSTEP>& rw, const LevelDesc& R, int view, int nsrc,
+                                                const float* wv, int x, int y, float4 h, float den) {
+    if (!(den > 0.f)) return 2.f;
+    float nm = 0.f;
+    const PairRec* prs = P.pairs + (size_t)view * MAXSRC;
+    const LevelPair* lps = P.lpairs + ((size_t)view * P.nlevels + P.level) * MAXSRC;
+#pragma nounroll
+    for (int j = 0; j < nsrc; ++j) {
+        float wj = wv[j * P.wv_stride];
+        if (!(wj > 0.f)) continue;
+        const PairRec& pr = prs[j];
+        const LevelDesc S = P.lv[pr.src * P.nlevels + P.level];
+        float g = 0.f;
+        float m = pair_cost<WR, WSTEP>(P, rw, R, S, pr, lps[j], x, y, h, &g);
+        nm += wj * (m + P.lambda * g);
+    }
+    return nm / den;
+}
+
+// ================================================================ half step (patchmatch.cu:439-657)
+template <int WR, int WSTEP>
+__global__ __launch_bounds__(256) void k_half(KParams P) {
+    constexpr int TW = HALF_SPAN + 2 * WR;
+    constexpr int TH = HALF_ROWS + 2 * WR;
+    __shared__ float tile[TW * TH];
+    extern __shared__ float wv_lds[];
+
+    BatchItem it;
+    int px0, py0;
+    if (!locate_tile(P, xcd_tile(blockIdx.x, P.ntiles), HALF_SPAN, HALF_ROWS, it, px0, py0)) return;
+    const LevelDesc R = P.lv[it.view * P.nlevels + P.level];
+    const int w = it.w, h = it.h;
+    {
+        const uint8_t* g = P.gray + R.goff;
+        const int tx0 = px0 - WR, ty0 = py0 - WR;
+        for (int i = threadIdx.x; i < TW * TH; i += 256) {
+            int ty = i / TW, tx = i - ty * TW;
+            int gx = tx0 + tx, gy = ty0 + ty;
+            tile[i] = (gx >= 0 && gy >= 0 && gx < w && gy < h) ? (float)g[(size_t)gy * w + gx] : 0.0f;
+        }
+    }
+    __syncthreads();
+    const int lane = threadIdx.x & 63, row = threadIdx.x >> 6;
+    const int y = py0 + row;
+    const int x = px0 + 2 * lane + ((y + P.color) & 1);
+    if (x >= w || y >= h) return;
+
+    RefTile<TW> ref{tile, px0 - WR, py0 - WR};
+    const ViewDesc vd = P.views[it.view];
+    const int nsrc = vd.nsrc;
+    const PairRec* prs = P.pairs + (size_t)it.view * MAXSRC;
+    const LevelPair* lps = P.lpairs + ((size_t)it.view * P.nlevels + P.level) * MAXSRC;
+    const long long base = it.off;
+    const int pix = y * w + x;
+    float4* pl = P.plane + base;
+    const float* co = P.cost + base;
+    const f3 ray = pixel_ray(R, x, y);
+    float* wv = wv_lds + threadIdx.x;
+
+    // 1. candidates: current + best of 8 regions (patchmatch.cu:477-511)
+    float4 hyp[9];
+    hyp[0] = pl[pix];
+#pragma unroll
+    for (int r = 0; r < 8; ++r) {
+        const int dir = r & 3;
+        const bool strip = r >= 4;
+        const int cnt = strip ? 11 : 7;
+        float best = 1e30f;
+        int bq = -1, bx = 0, by = 0;
+#pragma unroll
+        for (int k = 0; k < 11; ++k) {
+            if (k >= cnt) break;
+            int dx, dy;
+            if (strip) rot_dir(dir, 0, -(3 + 2 * k), dx, dy);
+            else rot_dir(dir, V_OFF[k][0], V_OFF[k][1], dx, dy);
+            int qx = x + dx, qy = y + dy;
+            if (qx < 0 || qy < 0 || qx >= w || qy >= h) continue;
+            float cq = co[qy * w + qx];
+            if (cq < best) {
+                best = cq;
+                bq = qy * w + qx;
+                bx = qx;
+                by = qy;
+            }
+        }
+        hyp[1 + r] = hyp[0];
+        if (bq >= 0) {
+            float4 q = pl[bq];
+            f3 n = xyz(q);
+            f3 rq = pixel_ray(R, bx, by);
+            float den = dot3(n, ray);
+            if (den < -1e-6f && q.w > 0.f) {
+                float d = q.w * dot3(n, rq) / den;
+                if (d > 0.f && isfinite(d)) hyp[1 + r] = mk4(n, d);
+            }
+        }
+    }
+
+    // 2. view selection and aggregation (patchmatch.cu:513-594)
+    RefWin<WR, WSTEP> rw;
+    ref_stats<WR, WSTEP>(P, ref, x, y, rw);
+    const int vprev = P.bview[base + pix];
+    float num[9], numf[9];
+#pragma unroll
+    for (int i = 0; i < 9; ++i) num[i] = numf[i] = 0.f;
+    float den = 0.f, wbest = 0.f, denf = 0.f;
+    int vbest = 255;
+#pragma nounroll
+    for (int j = 0; j < nsrc; ++j) {
+        const PairRec& pr = prs[j];
+        const LevelDesc S = P.lv[pr.src * P.nlevels + P.level];
+        const LevelPair& lp = lps[j];
+        float prior;
+        {
+            float H[9];
+            bool ok = homography(R, lp, (float)x, (float)y, hyp[0].w, xyz(hyp[0]), H);
+            prior = view_prior<WR>(P, R, pr, H, ok, x, y, hyp[0]);
+        }
+        wv[j * P.wv_stride] = 0.f;
+        if (!(prior > 0.f)) continue;
+        float m[9], gg[9];
+#pragma nounroll
+        for (int i = 0; i < 9; ++i) {
+            float4 hi = pick4(hyp, i);
+            float H[9];
+            bool ok = homography(R, lp, (float)x, (float)y, hi.w, xyz(hi), H);
+            float mi = ok ? ncc_cost<WR, WSTEP>(P, rw, S, H, x, y) : 2.f;
+            float gi = P.geometric ? fminf(geo_error(P, R, S, pr, x, y, hi.w), P.gmax) : 0.f;
+#pragma unroll
+            for (int q = 0; q < 9; ++q)
+                if (q == i) {
+                    m[q] = mi;
+                    gg[q] = gi;
+                }
+        }
+        int good = 0, bad = 0;
+        float conf = 0.f;
+#pragma unroll
+        for (int i = 1; i < 9; ++i) {
+            if (m[i] < P.tau_good) {
+                good++;
+                conf += c3d_expf(-m[i] * m[i] * P.inv2beta2);
+            }
+            if (m[i] > P.tau_bad) bad++;
+        }
+        bool adopt = good > P.n1 && bad < P.n2;
+        bool isprev = j == vprev;
+        float wp = adopt ? (isprev ? 2.f : 1.f) * conf / (float)good : (isprev ? P.prev_w : 0.f);
+        float w2 = wp * prior;
+        wv[j * P.wv_stride] = w2 > 0.f ? w2 : -prior;
+#pragma unroll
+        for (int i = 0; i < 9; ++i) numf[i] += prior * (m[i] + P.lambda * gg[i]);
+        denf += prior;
+        if (w2 > 0.f) {
+#pragma unroll
+            for (int i = 0; i < 9; ++i) num[i] += w2 * (m[i] + P.lambda * gg[i]);
+            den += w2;
+            if (w2 > wbest) {
+                wbest = w2;
+                vbest = j;
+            }
+        }
+    }
+    float agg[9];
+    int bi9 = 0;
+    if (den > 0.f) {
+#pragma unroll
+        for (int i = 0; i < 9; ++i) agg[i] = num[i] / den;
+#pragma unroll
+        for (int i = 1; i < 9; ++i)
+            if (agg[i] < agg[bi9]) bi9 = i;
+    } else if (denf > 0.f) {
+#pragma unroll
+        for (int i = 0; i < 9; ++i) agg[i] = numf[i] / denf;
+#pragma unroll
+        for (int i = 1; i < 9; ++i)
+            if (agg[i] < agg[bi9]) bi9 = i;
+        for (int j = 0; j < nsrc; ++j) wv[j * P.wv_stride] = -wv[j * P.wv_stride];
+        den = denf;
+    } else {
+#pragma unroll
+        for (int i = 0; i < 9; ++i) agg[i] = 2.f;
+    }
+    float4 best = pick4(hyp, bi9);
+    float bcost = 2.f;
+#pragma unroll
+    for (int q = 0; q < 9; ++q)
+        if (q == bi9) bcost = agg[q];
+
+    // 3. refinement: 6 hypotheses, plus the coarse prior at weak texture (patchmatch.cu:596-653)
+    unsigned long long st = rng_state(P.seed, vd.key, P.level, P.run_id, P.t, P.color, (unsigned long long)pix);
+    float d_r = random_depth(st, vd.dmin, vd.dmax);
+    f3 n_r = random_normal(st, ray);
+    float u = 2.f * rnd(st) - 1.f;
+    float inv = best.w > 0.f ? (1.f / best.w) * (1.f + u * P.eps) : -1.f;
+    float d_p = inv > 0.f ? 1.f / inv : d_r;
+    if (!(best.w > 0.f)) best.w = d_r;
+    f3 n0 = xyz(best);
+    f3 n_p = perturb_normal(st, n0, ray, P.phi);
+    float4 cand[7] = {mk4(n0, d_r), mk4(n_r, best.w), mk4(n_r, d_r), mk4(n0, d_p), mk4(n_p, best.w), mk4(n_p, d_p),
+                      make_float4(0.f, 0.f, 0.f, 0.f)};
+    int ncand = 6;
+    if (P.use_prior && rw.var < P.weak_var255) {
+        float4 q = P.prior[base + pix];
+        if (q.w > 0.f && dot3(xyz(q), ray) < 0.f) {
+            cand[6] = q;
+            ncand = 7;
+        }
+    }
+#pragma nounroll
+    for (int c = 0; c < ncand; ++c) {
+        float4 hc = pick4(cand, c);
+        float cc = candidate_cost<WR, WSTEP>(P, rw, R, it.view, nsrc, wv, x, y, hc, den);
+        if (cc < bcost) {
+            bcost = cc;
+            best = hc;
+        }
+    }
+    pl[pix] = best;
+    P.cost[base + pix] = bcost;
+    P.bview[base + pix] = (uint8_t)vbest;
+}
+
+// ================================================================ run start / evaluation (patchmatch.cu:659-692)
+template <int WR, int WSTEP>
+__global__ __launch_bounds__(256) void k_eval(KParams P, int mode) {
+    BatchItem it;
+    int px0, py0;
+    if (!locate_tile(P, xcd_tile(blockIdx.x, P.ntiles), PIX_TW, PIX_TH, it, px0, py0)) return;
+    const int x = px0 + (threadIdx.x & 63), y = py0 + (threadIdx.x >> 6);
+    if (x >= it.w || y >= it.h) return;
+    const LevelDesc R = P.lv[it.view * P.nlevels + P.level];
+    const ViewDesc vd = P.views[it.view];
+    const PairRec* prs = P.pairs + (size_t)it.view * MAXSRC;
+    const LevelPair* lps = P.lpairs + ((size_t)it.view * P.nlevels + P.level) * MAXSRC;
+    const int pix = y * it.w + x;
+    const long long base = it.off;
+    f3 ray = pixel_ray(R, x, y);
+    float4 h = P.plane[base + pix];
+    if (mode == 1 && P.random_init) {
+        unsigned long long st = rng_state(P.seed, vd.key, P.level, P.run_id, 0xFFFF, 0, (unsigned long long)pix);
+        float d = random_depth(st, vd.dmin, vd.dmax);
+        f3 n = random_normal(st, ray);
+        h = mk4(n, d);
+        P.plane[base + pix] = h;
+    }
+    float c = 2.f;
+    if (h.w > 0.f) {
+        RefGlobal g{P.gray + R.goff, it.w, it.h};
+        RefWin<WR, WSTEP> rw;
+        ref_stats<WR, WSTEP>(P, g, x, y, rw);
+        // top-K mean (patchmatch.cu:326-345)
+        float b0 = 1e30f, b1 = 1e30f, b2 = 1e30f;
+#pragma nounroll
+        for (int j = 0; j < vd.nsrc; ++j) {
+            const PairRec& pr = prs[j];
+            const LevelDesc S = P.lv[pr.src * P.nlevels + P.level];
+            float gq = 0.f;
+            float cc = pair_cost<WR, WSTEP>(P, rw, R, S, pr, lps[j], x, y, h, &gq);
+            if (P.geometric) cc = cc + P.lambda * gq;
+            if (cc < b0) {
+                b2 = b1;
+                b1 = b0;
+                b0 = cc;
+            } else if (cc < b1) {
+                b2 = b1;
+                b1 = cc;
+            } else if (cc < b2) {
+                b2 = cc;
+            }
+        }
+        int k = P.topk < vd.nsrc ? P.topk : vd.nsrc;
+        float s = b0;
+        if (k >= 2) s += b1;
+        if (k >= 3) s += b2;
+        c = k > 0 ? s / (float)k : 2.0f;
+    }
+    P.cost[base + pix] = c;
+    if (mode == 1) P.bview[base + pix] = 255;
+}
+
+// ================================================================ final read-out (patchmatch.cu:694-737)
+template <int WR, int WSTEP>
+__global__ __launch_bounds__(256) void k_filter(KParams P) {
+    BatchItem it;
+    int px0, py0;
+    if (!locate_tile(P, xcd_tile(blockIdx.x, P.ntiles), PIX_TW, PIX_TH, it, px0, py0)) return;
+    const int x = px0 + (threadIdx.x & 63), y = py0 + (threadIdx.x >> 6);
+    if (x >= it.w || y >= it.h) return;
+    const LevelDesc R = P.lv[it.view * P.nlevels + P.level];
+    const ViewDesc vd = P.views[it.view];
+    const PairRec* prs = P.pairs + (size_t)it.view * MAXSRC;
+    const LevelPair* lps = P.lpairs + ((size_t)it.view * P.nlevels + P.level) * MAXSRC;
+    const int pix = y * it.w + x;
+    const long long base = it.off;
+    float4 h = P.plane[base + pix];
+    int count = 0;
+    if (h.w > 0.f) {
+        RefGlobal g{P.gray + R.goff, it.w, it.h};
+        RefWin<WR, WSTEP> rw;
+        ref_stats<WR, WSTEP>(P, g, x, y, rw);
+        f3 ray = pixel_ray(R, x, y);
+        f3 X = mk3(ray.x * h.w, ray.y * h.w, h.w);
+        float lx = sqrtf(dot3(X, X));
+#pragma nounroll
+        for (int j = 0; j < vd.nsrc; ++j) {
+            const PairRec& pr = prs[j];
+            const LevelDesc S = P.lv[pr.src * P.nlevels + P.level];
+            f3 Sv = mk3(pr.c[0] - X.x, pr.c[1] - X.y, pr.c[2] - X.z);
+            float ls = sqrtf(dot3(Sv, Sv));
+            if (!(ls > 0.f)) continue;
+            float cost = -dot3(Sv, X) / (ls * lx);
+            if (cost > P.cos_filter_tri) continue;
+            float cphi = dot3(Sv, xyz(h)) / ls;
+            if (cphi <= 0.f) continue;
+            float H[9];
+            if (!homography(R, lps[j], (float)x, (float)y, h.w, xyz(h), H)) continue;
+            float m = ncc_cost<WR, WSTEP>(P, rw, S, H, x, y);
+            float e = P.ncc_norm * c3d_expf(-m * m * P.inv2ncc);
+            if (e / (e + 0.5f) < P.q_min) continue;
+            if (geo_error(P, R, S, pr, x, y, h.w) > P.filter_gmax) continue;
+            count++;
+        }
+    }
+    P.count[base + pix] = (uint8_t)(count < 255 ? count : 255);
+}
+
+// ================================================================ cost probe (no reference counterpart)
+template <int WR, int WSTEP>
+__global__ __launch_bounds__(256) void k_pair_costs(KParams P, int view, int slot, int n, const int* xy, const float4* planes,
+                                                    float* out) {
+    int i = blockIdx.x * 256 + threadIdx.x;
+    if (i >= n) return;
+    const LevelDesc R = P.lv[view * P.nlevels + P.level];
+    const PairRec& pr = P.pairs[(size_t)view * MAXSRC + slot];
+    const LevelPair& lp = P.lpairs[((size_t)view * P.nlevels + P.level) * MAXSRC + slot];
+    const LevelDesc S = P.lv[pr.src * P.nlevels + P.level];
+    int x = xy[2 * i], y = xy[2 * i + 1];
+    RefGlobal g{P.gray + R.goff, R.w, R.h};
+    RefWin<WR, WSTEP> rw;
+    ref_stats<WR, WSTEP>(P, g, x, y, rw);
+    float4 h = planes[i];
+    float H[9];
+    float c = 2.f;
+    if (homography(R, lp, (float)x, (float)y, h.w, xyz(h), H)) c = ncc_cost<WR, WSTEP>(P, rw, S, H, x, y);
+    out[i] = c;
+}
+
+// ================================================================ joint bilateral upsampling (patchmatch.cu:739-801)
+__global__ __launch_bounds__(256) void k_upsample(KParams P) {
+    BatchItem it;
+    int px0, py0;
+    if (!locate_tile(P, xcd_tile(blockIdx.x, P.ntiles), PIX_TW, PIX_TH, it, px0, py0)) return;
+    const int x = px0 + (threadIdx.x & 63), y = py0 + (threadIdx.x >> 6);
+    if (x >= it.w || y >= it.h) return;
+    const LevelDesc R = P.lv[it.view * P.nlevels + P.level];
+    const LevelDesc L = P.lv[it.view * P.nlevels + P.level - 1];
+    const float4* low = P.prior + it.low_off;
+    const f3 rp = pixel_ray(R, x, y);
+    const float ip = (float)P.gray[R.goff + (size_t)y * R.w + x] * (1.0f / 255.0f);
+    const float px = ((float)x - 0.5f) * 0.5f, py = ((float)y - 0.5f) * 0.5f;
+    const int qx0 = (int)roundf(px), qy0 = (int)roundf(py);
+    float sw = 0.f, sd = 0.f;
+    f3 sn = mk3(0.f, 0.f, 0.f);
+    float nd = 1e30f, ndep = 0.f;
+    f3 nn = mk3(0.f, 0.f, 0.f);
+    for (int qy = qy0 - 2; qy <= qy0 + 2; ++qy) {
+        if (qy < 0 || qy >= L.h) continue;
+        for (int qx = qx0 - 2; qx <= qx0 + 2; ++qx) {
+            if (qx < 0 || qx >= L.w) continue;
+            float4 q = low[qy * L.w + qx];
+            if (!(q.w > 0.f)) continue;
+            f3 nq = xyz(q);
+            float den = dot3(nq, rp);
+            if (!(den < -1e-6f)) continue;
+            float d = q.w * dot3(nq, pixel_ray(L, qx, qy)) / den;
+            if (!(d > 0.f) || !isfinite(d)) continue;
+            float ds = ((float)qx - px) * ((float)qx - px) + ((float)qy - py) * ((float)qy - py);
+            if (ds < nd) {
+                nd = ds;
+                ndep = d;
+                nn = nq;
+            }
+            float iq = (float)P.gray[L.goff + (size_t)qy * L.w + qx] * (1.0f / 255.0f);
+            float wgt = c3d_expf(-ds * P.up_inv2s - (ip - iq) * (ip - iq) * P.up_inv2c);
+            sw += wgt;
+            sd += wgt * d;
+            sn.x += wgt * nq.x;
+            sn.y += wgt * nq.y;
+            sn.z += wgt * nq.z;
+        }
+    }
+    float4 o = make_float4(0.f, 0.f, 0.f, 0.f);
+    float d = 0.f;
+    f3 nv = mk3(0.f, 0.f, 0.f);
+    bool ok = false;
+    if (sw > 1e-12f) {
+        d = sd / sw;
+        nv = sn;
+        ok = true;
+    } else if (nd < 1e29f) {
+        d = ndep;
+        nv = nn;
+        ok = true;
+    }
+    if (ok && dot3(nv, nv) > 1e-24f) {
+        nv = nrm3(nv);
+        if (dot3(nv, rp) >= 0.f) nv = mk3(-nv.x, -nv.y, -nv.z);
+        o = mk4(nv, d);
+    }
+    P.plane[it.off + y * it.w + x] = o;
+}
+
+// ================================================================ 5x5 median plane filter (patchmatch.cu:803-849)
+__global__ __launch_bounds__(256) void k_median(KParams P) {
+    BatchItem it;
+    int px0, py0;
+    if (!locate_tile(P, xcd_tile(blockIdx.x, P.ntiles), PIX_TW, PIX_TH, it, px0, py0)) return;
+    const int x = px0 + (threadIdx.x & 63), y = py0 + (threadIdx.x >> 6);
+    if (x >= it.w || y >= it.h) return;
+    const LevelDesc R = P.lv[it.view * P.nlevels + P.level];
+    const float4* in = P.prior + it.off;
+    const int w = it.w, h = it.h;
+    float4 self = in[y * w + x];
+    float4 out = self;
+    if (self.w > 0.f) {
+        const f3 rp = pixel_ray(R, x, y);
+        float ds[25];
+        int qs[25];
+        int c = 0;
+        int ya = y - 2 > 0 ? y - 2 : 0, yb = y + 3 < h ? y + 3 : h;
+        int xa = x - 2 > 0 ? x - 2 : 0, xb = x + 3 < w ? x + 3 : w;
+        for (int qy = ya; qy < yb; ++qy)
+            for (int qx = xa; qx < xb; ++qx) {
+                float4 q = in[qy * w + qx];
+                if (!(q.w > 0.f)) continue;
+                f3 nq = xyz(q);
+                float den = dot3(nq, rp);
+                if (!(den < -1e-6f)) continue;
+                float d = q.w * dot3(nq, pixel_ray(R, qx, qy)) / den;
+                if (!(d > 0.f) || !isfinite(d)) continue;
+                int qi = qy * w + qx;
+                int k = c++;
+                while (k > 0 && (ds[k - 1] > d || (ds[k - 1] == d && qs[k - 1] > qi))) {
+                    ds[k] = ds[k - 1];
+                    qs[k] = qs[k - 1];
+                    --k;
+                }
+                ds[k] = d;
+                qs[k] = qi;
+            }
+        if (c > 0) {
+            int m = (c - 1) / 2;
+            float4 q = in[qs[m]];
+            out = make_float4(q.x, q.y, q.z, ds[m]);
+        }
+    }
+    P.plane[it.off + y * w + x] = out;
+}
+
+// ================================================================ small device-side helpers
+
+// Quad tables of one view-level: entry (x0, y0), x0 in [-2, w], y0 in [-2, h].
+__global__ void k_quad_build(const uint8_t* g, int w, int h, uint32_t* q) {
+    int i = blockIdx.x * 256 + threadIdx.x;
+    int qp = w + 3;
+    if (i >= qp * (h + 3)) return;
+    int y0 = i / qp - 2, x0 = i % qp - 2;
+    auto at = [&](int x, int y) -> uint32_t {
+        return (x >= 0 && y >= 0 && x < w && y < h) ? (uint32_t)g[(size_t)y * w + x] : 0u;
+    };
+    q[i] = at(x0, y0) | (at(x0 + 1, y0) << 8) | (at(x0, y0 + 1) << 16) | (at(x0 + 1, y0 + 1) << 24);
+}
+
+// depth channel of the planes -> snapshot buffer (same pixel layout)
+__global__ void k_extract_depth(const float4* plane, float* out, long long n) {
+    long long i = (long long)blockIdx.x * 256 + threadIdx.x;
+    if (i < n) out[i] = plane[i].w;
+}
+
+// read-out threshold: depth/normal kept where depth > 0 and count >= need (densify.rs:446-454)
+__global__ void k_threshold(KParams P, const uint8_t* need_per_item, float4* out) {
+    BatchItem it;
+    int px0, py0, item = 0;
+    if (!locate_tile(P, blockIdx.x, PIX_TW, PIX_TH, it, px0, py0, &item)) return;
+    const int x = px0 + (threadIdx.x & 63), y = py0 + (threadIdx.x >> 6);
+    if (x >= it.w || y >= it.h) return;
+    long long p = it.off + y * it.w + x;
+    float4 h = P.plane[p];
+    if (!(h.w > 0.f && P.count[p] >= need_per_item[item])) h = make_float4(0.f, 0.f, 0.f, 0.f);
+    out[p] = h;
+}
+
+// ================================================================ launchers
+
+int half_tiles(const BatchItem* items, int n) {
+    int t = 0;
+    for (int i = 0; i < n; ++i) t += ((items[i].w + HALF_SPAN - 1) / HALF_SPAN) * ((items[i].h + HALF_ROWS - 1) / HALF_ROWS);
+    return t;
+}
+int pix_tiles(const BatchItem* items, int n) {
+    int t = 0;
+    for (int i = 0; i < n; ++i) t += ((items[i].w + PIX_TW - 1) / PIX_TW) * ((items[i].h + PIX_TH - 1) / PIX_TH);
+    return t;
+}
+
+template <int WR, int WSTEP>
+static hipError_t launch_half(const KParams& P, hipStream_t s) {
+    size_t lds = (size_t)P.wv_stride * (P.max_nsrc > 0 ? P.max_nsrc : 1) * sizeof(float);
+    hipLaunchKernelGGL((k_half<WR, WSTEP>), dim3(P.ntiles), dim3(256), lds, s, P);
+    return hipGetLastError();
+}
+template <int WR, int WSTEP>
+static hipError_t launch_eval(const KParams& P, int mode, hipStream_t s) {
+    hipLaunchKernelGGL((k_eval<WR, WSTEP>), dim3(P.ntiles), dim3(256), 0, s, P, mode);
+    return hipGetLastError();
+}
+template <int WR, int WSTEP>
+static hipError_t launch_filter(const KParams& P, hipStream_t s) {
+    hipLaunchKernelGGL((k_filter<WR, WSTEP>), dim3(P.ntiles), dim3(256), 0, s, P);
+    return hipGetLastError();
+}
+template <int WR, int WSTEP>
+static hipError_t launch_pair_costs(const KParams& P, int view, int slot, int n, const int* xy, const float4* planes, float* out,
+                                    hipStream_t s) {
+    hipLaunchKernelGGL((k_pair_costs<WR, WSTEP>), dim3((n + 255) / 256), dim3(256), 0, s, P, view, slot, n, xy, planes, out);
+    return hipGetLastError();
+}
+template <int WR, int WSTEP>
+static const void* half_symbol() {
+    return reinterpret_cast<const void*>(&k_half<WR, WSTEP>);
+}
+
+template <int WR, int WSTEP>
+static constexpr KernelSet make_set() {
+    return KernelSet{WR, WSTEP, &launch_half<WR, WSTEP>, &launch_eval<WR, WSTEP>, &launch_filter<WR, WSTEP>,
+                     &launch_pair_costs<WR, WSTEP>, &half_symbol<WR, WSTEP>};
+}
+
+// Window variants compiled into the library (radius, step): the fast profile (5, 2) and the
+// variants of the reference's window report (patchmatch_gpu.rs:164) whose window holds at most
+// 49 samples, so that the bilateral weights stay in registers.
+static const KernelSet kSets[] = {make_set<5, 2>(), make_set<4, 2>(), make_set<3, 2>(), make_set<2, 2>(), make_set<3, 1>(),
+                                  make_set<2, 1>(), make_set<1, 1>()};
+
+const KernelSet* kernel_set(int wr, int wstep) {
+    for (const KernelSet& k : kSets)
+        if (k.wr == wr && k.wstep == wstep) return &k;
+    return nullptr;
+}
+
+hipError_t launch_upsample(const KParams& P, hipStream_t s) {
+    hipLaunchKernelGGL(k_upsample, dim3(P.ntiles), dim3(256), 0, s, P);
+    return hipGetLastError();
+}
+hipError_t launch_median(const KParams& P, hipStream_t s) {
+    hipLaunchKernelGGL(k_median, dim3(P.ntiles), dim3(256), 0, s, P);
+    return hipGetLastError();
+}
+hipError_t launch_quad_build(const uint8_t* g, int w, int h, uint32_t* q, hipStream_t s) {
+    int n = (w + 3) * (h + 3);
+    hipLaunchKernelGGL(k_quad_build, dim3((n + 255) / 256), dim3(256), 0, s, g, w, h, q);
+    return hipGetLastError();
+}
+hipError_t launch_extract_depth(const float4* plane, float* out, long long n, hipStream_t s) {
+    hipLaunchKernelGGL(k_extract_depth, dim3((unsigned)((n + 255) / 256)), dim3(256), 0, s, plane, out, n);
+    return hipGetLastError();
+}
+hipError_t launch_threshold(const KParams& P, const uint8_t* need, float4* out, hipStream_t s) {
+    hipLaunchKernelGGL(k_threshold, dim3(P.ntiles), dim3(256), 0, s, P, need, out);
+    return hipGetLastError();
+}
+
+}  // namespace c3d

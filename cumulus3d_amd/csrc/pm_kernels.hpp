@@ -1,0 +1,32 @@
+// pm_kernels.hpp — host-side interface of the kernels in pm_kernels.hip.
+#pragma once
+
+#include "pm_device.hpp"
+
+namespace c3d {
+
+// One compiled window variant (radius, step) of the window-templated kernels.
+struct KernelSet {
+    int wr, wstep;
+    hipError_t (*half)(const KParams&, hipStream_t);
+    hipError_t (*eval)(const KParams&, int mode, hipStream_t);
+    hipError_t (*filter)(const KParams&, hipStream_t);
+    hipError_t (*pair_costs)(const KParams&, int view, int slot, int n, const int* xy, const float4* planes, float* out,
+                             hipStream_t);
+    const void* (*half_symbol)();
+};
+
+// nullptr when the library has no kernels for this window
+const KernelSet* kernel_set(int wr, int wstep);
+
+// workgroup counts of the two tilings for a batch
+int half_tiles(const BatchItem* items, int n);
+int pix_tiles(const BatchItem* items, int n);
+
+hipError_t launch_upsample(const KParams& P, hipStream_t s);
+hipError_t launch_median(const KParams& P, hipStream_t s);
+hipError_t launch_quad_build(const uint8_t* g, int w, int h, uint32_t* q, hipStream_t s);
+hipError_t launch_extract_depth(const float4* plane, float* out, long long n, hipStream_t s);
+hipError_t launch_threshold(const KParams& P, const uint8_t* need, float4* out, hipStream_t s);
+
+}  // namespace c3d

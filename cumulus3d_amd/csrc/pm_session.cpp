@@ -1,0 +1,916 @@
+// pm_session.cpp — the C ABI of include/c3d_pm.h: backend (device) and session (device-resident
+// images, geometry, states) for the gfx950 PatchStereo kernels.
+//
+// Host-side behaviour follows the reference's CUDA backend host code:
+//   constants          crates/cuda/src/patchmatch.rs:496-521
+//   run schedule       crates/cuda/src/patchmatch.rs:752-799 (eval mode 1, then t x {colour 0, 1})
+//   per-iteration      crates/cuda/src/patchmatch.rs:767-779 (tau(t), eps_t, phi_t)
+//   evaluate / filter  crates/cuda/src/patchmatch.rs:801-824, 842-868
+//   snapshot           crates/cuda/src/patchmatch.rs:573-604 (re-upload only on id change)
+//   batching           crates/cuda/src/patchmatch.rs:606-625 (pixel and view budgets)
+//   errors             crates/cuda/src/patchmatch.rs:292-294, 366-371, 638-642, 753-756
+// Differences by design: every image is uploaded once as u8 plus a packed quad table (no texture
+// objects); all views of a batch run in one flattened grid; c3d_pm_densify keeps states on the
+// device for the whole single-level schedule.
+#include <hip/hip_runtime.h>
+
+#include <algorithm>
+#include <chrono>
+#include <cmath>
+#include <cstdio>
+#include <cstring>
+#include <mutex>
+#include <string>
+#include <vector>
+
+#include "../../include/c3d_pm.h"
+#include "pm_kernels.hpp"
+
+using namespace c3d;
+
+namespace {
+
+thread_local std::string g_error;
+
+int fail(int code, const std::string& msg) {
+    g_error = msg;
+    return code;
+}
+
+#define HIP_TRY(expr)                                                                       \
+    do {                                                                                    \
+        hipError_t e_ = (expr);                                                             \
+        if (e_ != hipSuccess) return fail(C3D_DEVICE, std::string("HIP: ") + #expr + ": " + \
+                                                          hipGetErrorString(e_));           \
+    } while (0)
+
+const float kPi = 3.14159265358979323846f;
+inline float deg2rad(float d) { return d * (kPi / 180.0f); }
+
+// math.rs:49-76 (double)
+double erf_series(double x) {
+    if (std::fabs(x) > 5.0) return x > 0 ? 1.0 : -1.0;
+    double sum = 0.0, term = x, x2 = x * x;
+    for (int n = 0; n < 200; ++n) {
+        double t = term / (double)(2 * n + 1);
+        sum += t;
+        double as = std::fabs(sum) > 1e-300 ? std::fabs(sum) : 1e-300;
+        if (std::fabs(t) < 1e-17 * as) break;
+        term *= -x2 / (double)(n + 1);
+    }
+    return sum * 2.0 / std::sqrt(M_PI);
+}
+double emission_norm(double s) { return 2.0 / (std::sqrt(2.0 * M_PI) * s * erf_series(2.0 / (s * M_SQRT2))); }
+double emission(double c, double s) { return emission_norm(s) * std::exp(-c * c / (2.0 * s * s)); }
+
+// A = K_j R K^-1, b = K_j t (patchmatch.rs:131-151)
+LevelPair level_pair(const float kr[4], const float kj[4], const float r[9], const float t[3]) {
+    double fx = kr[0], fy = kr[1], cx = kr[2], cy = kr[3];
+    double kinv[3][3] = {{1.0 / fx, 0.0, -cx / fx}, {0.0, 1.0 / fy, -cy / fy}, {0.0, 0.0, 1.0}};
+    double kjm[3][3] = {{kj[0], 0.0, kj[2]}, {0.0, kj[1], kj[3]}, {0.0, 0.0, 1.0}};
+    double kr_[3][3];
+    for (int i = 0; i < 3; ++i)
+        for (int k = 0; k < 3; ++k) {
+            double s = 0.0;
+            for (int m = 0; m < 3; ++m) s += kjm[i][m] * (double)r[m * 3 + k];
+            kr_[i][k] = s;
+        }
+    LevelPair L;
+    for (int i = 0; i < 3; ++i)
+        for (int k = 0; k < 3; ++k) {
+            double s = 0.0;
+            for (int m = 0; m < 3; ++m) s += kr_[i][m] * kinv[m][k];
+            L.A[i * 3 + k] = (float)s;
+        }
+    for (int i = 0; i < 3; ++i) {
+        double s = 0.0;
+        for (int m = 0; m < 3; ++m) s += kjm[i][m] * (double)t[m];
+        L.b[i] = (float)s;
+    }
+    return L;
+}
+
+template <class T>
+struct DevBuf {
+    T* p = nullptr;
+    size_t n = 0;
+    hipError_t reserve(size_t want) {
+        if (want <= n && p) return hipSuccess;
+        if (p) (void)hipFree(p);
+        p = nullptr;
+        n = 0;
+        hipError_t e = hipMalloc((void**)&p, std::max<size_t>(want, 1) * sizeof(T));
+        if (e == hipSuccess) n = std::max<size_t>(want, 1);
+        return e;
+    }
+    void release() {
+        if (p) (void)hipFree(p);
+        p = nullptr;
+        n = 0;
+    }
+};
+
+}  // namespace
+
+struct c3d_pm_backend {
+    int device = 0;
+    std::string name;
+    c3d_pm_options opts{};
+    std::mutex lock;
+    hipStream_t stream = nullptr;
+};
+
+struct c3d_pm_session {
+    c3d_pm_backend* be = nullptr;
+    std::unique_lock<std::mutex> guard;
+    const KernelSet* ks = nullptr;
+    int nv = 0, nl = 0;
+    std::vector<LevelDesc> lv;
+    std::vector<ViewDesc> vd;
+    // device tables
+    DevBuf<uint8_t> gray;
+    DevBuf<uint32_t> quad;
+    DevBuf<LevelDesc> d_lv;
+    DevBuf<PairRec> d_pairs;
+    DevBuf<LevelPair> d_lpairs;
+    DevBuf<ViewDesc> d_views;
+    DevBuf<BatchItem> d_batch;
+    // snapshot
+    bool snap_valid = false;
+    uint64_t snap_id = 0;
+    int snap_level = -1;
+    DevBuf<float> snap;
+    DevBuf<long long> snap_off;
+    // states
+    DevBuf<float4> plane, prior, plane2;
+    DevBuf<float> cost;
+    DevBuf<uint8_t> bview, count, need;
+    // probes
+    DevBuf<int> q_xy;
+    DevBuf<float4> q_planes;
+    DevBuf<float> q_out;
+    KParams base{};
+    c3d_pm_params pm_copy{};
+    c3d_filter_params filter_copy{};
+    std::vector<float4> host4;
+    std::vector<float> host1;
+    hipEvent_t ev[8] = {};
+    // per-launch timing of the half-step kernels: pairs of events, read after the final sync
+    std::vector<hipEvent_t> pool;
+    size_t pool_used = 0;
+};
+
+namespace {
+
+int set_device(c3d_pm_backend* be) {
+    HIP_TRY(hipSetDevice(be->device));
+    return C3D_OK;
+}
+
+// States of `views` at `level` must match the level size (patchmatch.rs:638-642).
+int check_views(c3d_pm_session* s, uint32_t level, const uint32_t* views, uint32_t n, const c3d_view_state* st) {
+    if ((int)level >= s->nl) return fail(C3D_INVALID_ARGUMENT, "level " + std::to_string(level) + " out of range");
+    for (uint32_t i = 0; i < n; ++i) {
+        if ((int)views[i] >= s->nv) return fail(C3D_INVALID_ARGUMENT, "view index out of range");
+        if (st) {
+            const LevelDesc& L = s->lv[views[i] * s->nl + level];
+            if ((int)st[i].width != L.w || (int)st[i].height != L.h)
+                return fail(C3D_INVALID_ARGUMENT, "state size " + std::to_string(st[i].width) + "x" + std::to_string(st[i].height) +
+                                                      " != level " + std::to_string(L.w) + "x" + std::to_string(L.h));
+            if (!st[i].depth || !st[i].normal) return fail(C3D_INVALID_ARGUMENT, "state without depth/normal buffers");
+        }
+    }
+    return C3D_OK;
+}
+
+// Bundles views by the pixel and view budgets (patchmatch.rs:606-625): [start, end) pairs.
+std::vector<std::pair<uint32_t, uint32_t>> batches(c3d_pm_session* s, const uint32_t* views, uint32_t n, uint32_t level) {
+    std::vector<std::pair<uint32_t, uint32_t>> out;
+    long long budget = s->be->opts.batch_pixels > 0 ? s->be->opts.batch_pixels : (1LL << 40);
+    uint32_t maxb = s->be->opts.max_batch > 0 ? (uint32_t)s->be->opts.max_batch : 64u;
+    uint32_t start = 0;
+    long long pix = 0;
+    for (uint32_t i = 0; i < n; ++i) {
+        const LevelDesc& L = s->lv[views[i] * s->nl + level];
+        long long p = (long long)L.w * L.h;
+        if (i > start && (pix + p > budget || i - start >= maxb)) {
+            out.push_back({start, i});
+            start = i;
+            pix = 0;
+        }
+        pix += p;
+    }
+    if (start < n) out.push_back({start, n});
+    return out;
+}
+
+// Writes batch items for views[a..b) at `level` to the device; returns the pixel count.
+int set_batch(c3d_pm_session* s, const uint32_t* views, uint32_t a, uint32_t b, uint32_t level, std::vector<BatchItem>& items,
+              long long& npix, int& max_nsrc) {
+    items.clear();
+    npix = 0;
+    max_nsrc = 1;
+    for (uint32_t i = a; i < b; ++i) {
+        const LevelDesc& L = s->lv[views[i] * s->nl + level];
+        BatchItem it{};
+        it.off = npix;
+        it.view = (int)views[i];
+        it.w = L.w;
+        it.h = L.h;
+        items.push_back(it);
+        npix += (long long)L.w * L.h;
+        max_nsrc = std::max(max_nsrc, s->vd[views[i]].nsrc);
+    }
+    HIP_TRY(s->d_batch.reserve(items.size()));
+    HIP_TRY(hipMemcpyAsync(s->d_batch.p, items.data(), items.size() * sizeof(BatchItem), hipMemcpyHostToDevice, s->be->stream));
+    HIP_TRY(s->plane.reserve(npix));
+    HIP_TRY(s->cost.reserve(npix));
+    HIP_TRY(s->bview.reserve(npix));
+    HIP_TRY(s->count.reserve(npix));
+    return C3D_OK;
+}
+
+KParams params_for(c3d_pm_session* s, uint32_t level, bool geometric, const std::vector<BatchItem>& items, int max_nsrc) {
+    KParams k = s->base;
+    k.gray = s->gray.p;
+    k.quad = s->quad.p;
+    k.lv = s->d_lv.p;
+    k.pairs = s->d_pairs.p;
+    k.lpairs = s->d_lpairs.p;
+    k.views = s->d_views.p;
+    k.batch = s->d_batch.p;
+    k.snap = s->snap.p;
+    k.snap_off = s->snap_off.p;
+    k.plane = s->plane.p;
+    k.cost = s->cost.p;
+    k.bview = s->bview.p;
+    k.count = s->count.p;
+    k.prior = s->prior.p;
+    k.nitems = (int)items.size();
+    k.level = (int)level;
+    k.geometric = geometric ? 1 : 0;
+    k.wv_stride = 256;
+    k.max_nsrc = max_nsrc;
+    return k;
+}
+
+// Pack host states (SoA) into float4 planes at the item offsets (patchmatch.rs:327-336).
+void pack(c3d_pm_session* s, const c3d_view_state* st, const std::vector<BatchItem>& items, long long npix, bool prior) {
+    s->host4.resize((size_t)npix);
+    for (size_t i = 0; i < items.size(); ++i) {
+        const c3d_view_state& v = st[i];
+        float4* o = s->host4.data() + items[i].off;
+        size_t n = (size_t)v.width * v.height;
+        if (prior) {
+            std::memcpy(o, v.prior, n * sizeof(float4));
+        } else {
+            for (size_t p = 0; p < n; ++p) o[p] = make_float4(v.normal[3 * p], v.normal[3 * p + 1], v.normal[3 * p + 2], v.depth[p]);
+        }
+    }
+}
+
+void unpack(const std::vector<float4>& h, const std::vector<BatchItem>& items, c3d_view_state* st) {
+    for (size_t i = 0; i < items.size(); ++i) {
+        c3d_view_state& v = st[i];
+        const float4* p = h.data() + items[i].off;
+        size_t n = (size_t)v.width * v.height;
+        for (size_t q = 0; q < n; ++q) {
+            v.normal[3 * q] = p[q].x;
+            v.normal[3 * q + 1] = p[q].y;
+            v.normal[3 * q + 2] = p[q].z;
+            v.depth[q] = p[q].w;
+        }
+    }
+}
+
+// Snapshot upload, only when the id changes (patchmatch.rs:573-604).
+int ensure_snapshot(c3d_pm_session* s, const c3d_depth_snapshot* snap) {
+    if (!snap) return fail(C3D_INVALID_ARGUMENT, "a geometric run needs a depth snapshot");
+    if ((int)snap->level >= s->nl) return fail(C3D_INVALID_ARGUMENT, "snapshot level out of range");
+    if (s->snap_valid && s->snap_id == snap->id) return C3D_OK;
+    std::vector<long long> offs((size_t)s->nv, -1);
+    size_t total = 0;
+    for (int v = 0; v < s->nv && (uint32_t)v < snap->num_maps; ++v) {
+        if (snap->maps[v]) {
+            const LevelDesc& L = s->lv[v * s->nl + snap->level];
+            offs[v] = (long long)total;
+            total += (size_t)L.w * L.h;
+        }
+    }
+    HIP_TRY(s->snap.reserve(std::max<size_t>(total, 1)));
+    HIP_TRY(s->snap_off.reserve((size_t)s->nv));
+    for (int v = 0; v < s->nv; ++v) {
+        if (offs[v] >= 0) {
+            const LevelDesc& L = s->lv[v * s->nl + snap->level];
+            HIP_TRY(hipMemcpyAsync(s->snap.p + offs[v], snap->maps[v], (size_t)L.w * L.h * sizeof(float), hipMemcpyHostToDevice,
+                                   s->be->stream));
+        }
+    }
+    HIP_TRY(hipMemcpyAsync(s->snap_off.p, offs.data(), offs.size() * sizeof(long long), hipMemcpyHostToDevice, s->be->stream));
+    HIP_TRY(hipStreamSynchronize(s->be->stream));
+    s->snap_valid = true;
+    s->snap_id = snap->id;
+    s->snap_level = (int)snap->level;
+    return C3D_OK;
+}
+
+// Per-iteration constants (patchmatch.rs:767-779).
+void iteration_constants(const c3d_pm_params& pm, bool geometric, uint32_t t, KParams& k) {
+    float tf = (float)t;
+    float eps0 = geometric ? pm.eps0_geometric : pm.eps0_photometric;
+    float phi0 = geometric ? pm.phi0_geometric_deg : pm.phi0_photometric_deg;
+    float p = 1.0f;
+    for (uint32_t i = 0; i < t && p > 0.0f; ++i) p *= 0.5f;
+    k.t = (int)t;
+    k.tau_good = pm.tau0 * std::exp(-tf * tf / pm.tau_alpha);
+    k.eps = std::fmax(eps0 * p, pm.eps_min);
+    k.phi = deg2rad(std::fmax(phi0 * p, pm.phi_min_deg));
+}
+
+// One run over the current batch: eval mode 1, then iterations x colours (patchmatch.rs:766-779).
+// With `timed`, every half-step launch is bracketed by a pair of pool events (read later).
+int launch_run(c3d_pm_session* s, KParams k, const std::vector<BatchItem>& items, const c3d_pm_params& pm, bool geometric,
+               uint32_t iterations, bool timed) {
+    hipStream_t st = s->be->stream;
+    k.ntiles = pix_tiles(items.data(), (int)items.size());
+    HIP_TRY(s->ks->eval(k, 1, st));
+    k.ntiles = half_tiles(items.data(), (int)items.size());
+    for (uint32_t t = 0; t < iterations; ++t) {
+        iteration_constants(pm, geometric, t, k);
+        for (int color = 0; color < 2; ++color) {
+            k.color = color;
+            if (timed) {
+                while (s->pool.size() < s->pool_used + 2) {
+                    hipEvent_t e;
+                    HIP_TRY(hipEventCreate(&e));
+                    s->pool.push_back(e);
+                }
+                HIP_TRY(hipEventRecord(s->pool[s->pool_used], st));
+            }
+            HIP_TRY(s->ks->half(k, st));
+            if (timed) {
+                HIP_TRY(hipEventRecord(s->pool[s->pool_used + 1], st));
+                s->pool_used += 2;
+            }
+        }
+    }
+    return C3D_OK;
+}
+
+// Sum of the recorded half-step launch times (after the stream has drained).
+int pool_sum(c3d_pm_session* s, double* ms_out, uint32_t* n_out) {
+    double tot = 0.0;
+    for (size_t i = 0; i + 1 < s->pool_used; i += 2) {
+        float ms = 0.f;
+        HIP_TRY(hipEventElapsedTime(&ms, s->pool[i], s->pool[i + 1]));
+        tot += ms;
+    }
+    *ms_out = tot;
+    *n_out = (uint32_t)(s->pool_used / 2);
+    s->pool_used = 0;
+    return C3D_OK;
+}
+
+}  // namespace
+
+// ================================================================ C ABI
+
+extern "C" {
+
+const char* c3d_pm_last_error(void) { return g_error.c_str(); }
+
+int c3d_pm_is_available(void) {
+    int n = 0;
+    if (hipGetDeviceCount(&n) != hipSuccess || n <= 0) return 0;
+    hipDeviceProp_t pr;
+    if (hipGetDeviceProperties(&pr, 0) != hipSuccess) return 0;
+    return std::strncmp(pr.gcnArchName, "gfx950", 6) == 0 ? 1 : 0;
+}
+
+int c3d_pm_create(int ordinal, const c3d_pm_options* opts, c3d_pm_backend** out) {
+    if (!out) return fail(C3D_INVALID_ARGUMENT, "null output handle");
+    *out = nullptr;
+    int n = 0;
+    if (hipGetDeviceCount(&n) != hipSuccess || n <= 0) return fail(C3D_NOT_AVAILABLE, "no HIP device");
+    if (ordinal < 0 || ordinal >= n) return fail(C3D_NOT_AVAILABLE, "device ordinal out of range");
+    hipDeviceProp_t pr;
+    HIP_TRY(hipGetDeviceProperties(&pr, ordinal));
+    if (std::strncmp(pr.gcnArchName, "gfx950", 6) != 0)
+        return fail(C3D_NOT_AVAILABLE, std::string("device is ") + pr.gcnArchName + ", this library is built for gfx950");
+    c3d_pm_backend* be = new c3d_pm_backend();
+    be->device = ordinal;
+    be->name = std::string("hip(") + (pr.name[0] ? pr.name : "AMD Instinct") + ", " + pr.gcnArchName + ")";
+    if (opts) be->opts = *opts;
+    if (be->opts.batch_pixels <= 0) be->opts.batch_pixels = 1LL << 40;
+    if (be->opts.max_batch <= 0) be->opts.max_batch = 64;
+    if (hipSetDevice(ordinal) != hipSuccess || hipStreamCreateWithFlags(&be->stream, hipStreamNonBlocking) != hipSuccess) {
+        delete be;
+        return fail(C3D_DEVICE, "HIP: cannot create a stream");
+    }
+    *out = be;
+    return C3D_OK;
+}
+
+const char* c3d_pm_name(const c3d_pm_backend* be) { return be ? be->name.c_str() : ""; }
+
+void c3d_pm_destroy(c3d_pm_backend* be) {
+    if (!be) return;
+    (void)hipSetDevice(be->device);
+    if (be->stream) (void)hipStreamDestroy(be->stream);
+    delete be;
+}
+
+int c3d_pm_begin(c3d_pm_backend* be, const c3d_kernel_input* in, c3d_pm_session** out) {
+    if (!be || !in || !out) return fail(C3D_INVALID_ARGUMENT, "null argument");
+    *out = nullptr;
+    const c3d_pm_params& pm = in->pm;
+    if (pm.window_radius < 1 || pm.window_radius > 20 || pm.window_step < 1 || pm.window_step > 2)
+        return fail(C3D_INVALID_ARGUMENT, "window radius " + std::to_string(pm.window_radius) + " (1..=20), step " +
+                                              std::to_string(pm.window_step) + " (1|2)");
+    const KernelSet* ks = kernel_set(pm.window_radius, pm.window_step);
+    if (!ks)
+        return fail(C3D_INVALID_ARGUMENT, "window radius " + std::to_string(pm.window_radius) + " step " + std::to_string(pm.window_step) +
+                                              " has no compiled kernels in this build");
+    if (in->num_levels < 1) return fail(C3D_INVALID_ARGUMENT, "num_levels must be >= 1");
+    int rc = set_device(be);
+    if (rc) return rc;
+    std::unique_lock<std::mutex> guard(be->lock);
+    const int nv = (int)in->num_views, nl = (int)in->num_levels;
+    // validate
+    for (int v = 0; v < nv; ++v) {
+        const c3d_kernel_view& kv = in->views[v];
+        if (kv.num_sources > C3D_MAX_SOURCES)
+            return fail(C3D_INVALID_ARGUMENT, "source views " + std::to_string(kv.num_sources) + " > 32");
+        for (uint32_t j = 0; j < kv.num_sources; ++j)
+            if ((int)kv.sources[j] >= nv || (int)kv.sources[j] == v)
+                return fail(C3D_INVALID_ARGUMENT, "source index out of range");
+        for (int l = 0; l < nl; ++l)
+            if (!kv.levels[l].gray || kv.levels[l].width < 1 || kv.levels[l].height < 1)
+                return fail(C3D_INVALID_ARGUMENT, "empty level image");
+    }
+    c3d_pm_session* s = new c3d_pm_session();
+    s->be = be;
+    s->ks = ks;
+    s->nv = nv;
+    s->nl = nl;
+    s->lv.resize((size_t)nv * nl);
+    s->vd.resize((size_t)nv);
+    // image layout
+    size_t gtot = 0, qtot = 0;
+    for (int v = 0; v < nv; ++v)
+        for (int l = 0; l < nl; ++l) {
+            const c3d_level_image& li = in->views[v].levels[l];
+            LevelDesc& L = s->lv[(size_t)v * nl + l];
+            L.w = (int)li.width;
+            L.h = (int)li.height;
+            L.qpitch = L.w + 3;
+            L.goff = (long long)gtot;
+            gtot += (size_t)L.w * L.h;
+            gtot = (gtot + 255) & ~(size_t)255;
+            L.qoff = (long long)qtot + 2LL * L.qpitch + 2;
+            qtot += (size_t)L.qpitch * (L.h + 3);
+            qtot = (qtot + 63) & ~(size_t)63;
+            L.fx = li.k[0];
+            L.fy = li.k[1];
+            L.cx = li.k[2];
+            L.cy = li.k[3];
+        }
+    auto bail = [&](int code) {
+        c3d_pm_end(s);
+        return code;
+    };
+    s->guard = std::move(guard);
+    if (hipEventCreate(&s->ev[0]) != hipSuccess) return bail(fail(C3D_DEVICE, "HIP: event"));
+    for (int i = 1; i < 8; ++i) (void)hipEventCreate(&s->ev[i]);
+    hipStream_t st = be->stream;
+    if (s->gray.reserve(gtot) != hipSuccess || s->quad.reserve(qtot) != hipSuccess)
+        return bail(fail(C3D_DEVICE, "HIP: out of device memory for images"));
+    for (int v = 0; v < nv; ++v)
+        for (int l = 0; l < nl; ++l) {
+            const c3d_level_image& li = in->views[v].levels[l];
+            const LevelDesc& L = s->lv[(size_t)v * nl + l];
+            if (hipMemcpyAsync(s->gray.p + L.goff, li.gray, (size_t)L.w * L.h, hipMemcpyHostToDevice, st) != hipSuccess)
+                return bail(fail(C3D_DEVICE, "HIP: image upload"));
+            if (launch_quad_build(s->gray.p + L.goff, L.w, L.h, s->quad.p + (L.qoff - 2LL * L.qpitch - 2), st) != hipSuccess)
+                return bail(fail(C3D_DEVICE, "HIP: quad build"));
+        }
+    std::vector<PairRec> prs((size_t)nv * MAXSRC);
+    std::vector<LevelPair> lps((size_t)nv * nl * MAXSRC);
+    std::memset(prs.data(), 0, prs.size() * sizeof(PairRec));
+    std::memset(lps.data(), 0, lps.size() * sizeof(LevelPair));
+    for (int v = 0; v < nv; ++v) {
+        const c3d_kernel_view& kv = in->views[v];
+        for (uint32_t j = 0; j < kv.num_sources; ++j) {
+            PairRec& p = prs[(size_t)v * MAXSRC + j];
+            p.src = (int)kv.sources[j];
+            std::memcpy(p.r, kv.pairs[j].r, sizeof p.r);
+            std::memcpy(p.t, kv.pairs[j].t, sizeof p.t);
+            std::memcpy(p.c, kv.pairs[j].center, sizeof p.c);
+            for (int l = 0; l < nl; ++l)
+                lps[((size_t)v * nl + l) * MAXSRC + j] = level_pair(kv.levels[l].k, in->views[p.src].levels[l].k, p.r, p.t);
+        }
+        ViewDesc& d = s->vd[v];
+        d.key = kv.key;
+        d.dmin = kv.depth_min;
+        d.dmax = kv.depth_max;
+        d.nsrc = (int)kv.num_sources;
+    }
+    if (s->d_lv.reserve(s->lv.size()) != hipSuccess || s->d_pairs.reserve(prs.size()) != hipSuccess ||
+        s->d_lpairs.reserve(lps.size()) != hipSuccess || s->d_views.reserve(s->vd.size()) != hipSuccess ||
+        s->snap_off.reserve((size_t)nv) != hipSuccess || s->snap.reserve(1) != hipSuccess)
+        return bail(fail(C3D_DEVICE, "HIP: out of device memory for tables"));
+    std::vector<long long> none((size_t)nv, -1);
+    if (hipMemcpyAsync(s->d_lv.p, s->lv.data(), s->lv.size() * sizeof(LevelDesc), hipMemcpyHostToDevice, st) != hipSuccess ||
+        hipMemcpyAsync(s->d_pairs.p, prs.data(), prs.size() * sizeof(PairRec), hipMemcpyHostToDevice, st) != hipSuccess ||
+        hipMemcpyAsync(s->d_lpairs.p, lps.data(), lps.size() * sizeof(LevelPair), hipMemcpyHostToDevice, st) != hipSuccess ||
+        hipMemcpyAsync(s->d_views.p, s->vd.data(), s->vd.size() * sizeof(ViewDesc), hipMemcpyHostToDevice, st) != hipSuccess ||
+        hipMemcpyAsync(s->snap_off.p, none.data(), none.size() * sizeof(long long), hipMemcpyHostToDevice, st) != hipSuccess ||
+        hipStreamSynchronize(st) != hipSuccess)
+        return bail(fail(C3D_DEVICE, "HIP: table upload"));
+    // constants (patchmatch.rs:496-521)
+    KParams& k = s->base;
+    std::memset(&k, 0, sizeof k);
+    float sc = pm.sigma_spatial > 0.0f ? pm.sigma_spatial : (float)pm.window_radius;
+    const c3d_filter_params& fl = in->filter;
+    k.seed = in->seed;
+    k.nlevels = nl;
+    k.topk = (int)pm.top_k;
+    k.tau_bad = pm.tau1;
+    k.inv2beta2 = 1.0f / (2.0f * pm.beta * pm.beta);
+    k.prev_w = pm.prev_view_weight;
+    k.n1 = (int)pm.n1;
+    k.n2 = (int)pm.n2;
+    k.inv2s = 1.0f / (2.0f * sc * sc);
+    float inv2c = 1.0f / (2.0f * pm.sigma_color * pm.sigma_color);
+    k.inv2c255 = inv2c / 65025.0f;
+    k.vmin255 = 1e-5f * 65025.0f;
+    k.weak_var255 = pm.weak_texture_var * 65025.0f;
+    k.cos_tri_min = std::cos(deg2rad(pm.min_triangulation_angle_deg));
+    k.inv2inc = 1.0f / (2.0f * pm.incident_angle_sigma * pm.incident_angle_sigma);
+    k.lambda = pm.geom_lambda;
+    k.gmax = pm.geom_max_cost;
+    k.cos_filter_tri = std::cos(deg2rad(fl.min_triangulation_angle_deg));
+    k.q_min = (float)emission(1.0 - (double)fl.min_ncc, (double)fl.ncc_sigma);
+    k.ncc_norm = (float)emission_norm((double)fl.ncc_sigma);
+    k.inv2ncc = 1.0f / (2.0f * fl.ncc_sigma * fl.ncc_sigma);
+    k.filter_gmax = fl.geom_max_cost;
+    s->pm_copy = pm;
+    s->filter_copy = fl;
+    *out = s;
+    return C3D_OK;
+}
+
+void c3d_pm_end(c3d_pm_session* s) {
+    if (!s) return;
+    (void)hipSetDevice(s->be->device);
+    (void)hipStreamSynchronize(s->be->stream);
+    s->gray.release();
+    s->quad.release();
+    s->d_lv.release();
+    s->d_pairs.release();
+    s->d_lpairs.release();
+    s->d_views.release();
+    s->d_batch.release();
+    s->snap.release();
+    s->snap_off.release();
+    s->plane.release();
+    s->prior.release();
+    s->plane2.release();
+    s->cost.release();
+    s->bview.release();
+    s->count.release();
+    s->need.release();
+    s->q_xy.release();
+    s->q_planes.release();
+    s->q_out.release();
+    for (auto& e : s->ev)
+        if (e) (void)hipEventDestroy(e);
+    for (auto& e : s->pool) (void)hipEventDestroy(e);
+    delete s;
+}
+
+int c3d_pm_run(c3d_pm_session* s, const uint32_t* views, uint32_t n, c3d_view_state* states, const c3d_run_params* p,
+               const c3d_depth_snapshot* snap) {
+    if (!s || !p || (n && (!views || !states))) return fail(C3D_INVALID_ARGUMENT, "null argument");
+    int rc = set_device(s->be);
+    if (rc) return rc;
+    rc = check_views(s, p->level, views, n, states);
+    if (rc) return rc;
+    if (p->geometric) {
+        rc = ensure_snapshot(s, snap);
+        if (rc) return rc;
+    }
+    hipStream_t st = s->be->stream;
+    for (auto [a, b] : batches(s, views, n, p->level)) {
+        std::vector<BatchItem> items;
+        long long npix;
+        int max_nsrc;
+        rc = set_batch(s, views, a, b, p->level, items, npix, max_nsrc);
+        if (rc) return rc;
+        bool use_prior = p->use_prior != 0;
+        for (uint32_t i = a; i < b; ++i)
+            if (!states[i].prior) use_prior = false;
+        if (!p->random_init) {
+            pack(s, states + a, items, npix, false);
+            HIP_TRY(hipMemcpyAsync(s->plane.p, s->host4.data(), (size_t)npix * sizeof(float4), hipMemcpyHostToDevice, st));
+            HIP_TRY(hipStreamSynchronize(st));
+        }
+        if (use_prior) {
+            HIP_TRY(s->prior.reserve(npix));
+            pack(s, states + a, items, npix, true);
+            HIP_TRY(hipMemcpyAsync(s->prior.p, s->host4.data(), (size_t)npix * sizeof(float4), hipMemcpyHostToDevice, st));
+            HIP_TRY(hipStreamSynchronize(st));
+        }
+        KParams k = params_for(s, p->level, p->geometric != 0, items, max_nsrc);
+        k.run_id = (int)p->run_id;
+        k.random_init = p->random_init ? 1 : 0;
+        k.use_prior = use_prior ? 1 : 0;
+        rc = launch_run(s, k, items, s->pm_copy, p->geometric != 0, p->iterations, false);
+        if (rc) return rc;
+        s->host4.resize((size_t)npix);
+        s->host1.resize((size_t)npix);
+        HIP_TRY(hipMemcpyAsync(s->host4.data(), s->plane.p, (size_t)npix * sizeof(float4), hipMemcpyDeviceToHost, st));
+        HIP_TRY(hipMemcpyAsync(s->host1.data(), s->cost.p, (size_t)npix * sizeof(float), hipMemcpyDeviceToHost, st));
+        HIP_TRY(hipStreamSynchronize(st));
+        unpack(s->host4, items, states + a);
+        for (size_t i = 0; i < items.size(); ++i)
+            if (states[a + i].cost)
+                std::memcpy(states[a + i].cost, s->host1.data() + items[i].off, (size_t)items[i].w * items[i].h * sizeof(float));
+    }
+    return C3D_OK;
+}
+
+int c3d_pm_evaluate(c3d_pm_session* s, uint32_t level, const uint32_t* views, uint32_t n, const c3d_view_state* states, int geometric,
+                    const c3d_depth_snapshot* snap, float* const* costs) {
+    if (!s || (n && (!views || !states || !costs))) return fail(C3D_INVALID_ARGUMENT, "null argument");
+    int rc = set_device(s->be);
+    if (rc) return rc;
+    rc = check_views(s, level, views, n, states);
+    if (rc) return rc;
+    if (geometric) {
+        rc = ensure_snapshot(s, snap);
+        if (rc) return rc;
+    }
+    hipStream_t st = s->be->stream;
+    for (auto [a, b] : batches(s, views, n, level)) {
+        std::vector<BatchItem> items;
+        long long npix;
+        int max_nsrc;
+        rc = set_batch(s, views, a, b, level, items, npix, max_nsrc);
+        if (rc) return rc;
+        pack(s, states + a, items, npix, false);
+        HIP_TRY(hipMemcpyAsync(s->plane.p, s->host4.data(), (size_t)npix * sizeof(float4), hipMemcpyHostToDevice, st));
+        KParams k = params_for(s, level, geometric != 0, items, max_nsrc);
+        k.ntiles = pix_tiles(items.data(), (int)items.size());
+        HIP_TRY(s->ks->eval(k, 0, st));
+        s->host1.resize((size_t)npix);
+        HIP_TRY(hipMemcpyAsync(s->host1.data(), s->cost.p, (size_t)npix * sizeof(float), hipMemcpyDeviceToHost, st));
+        HIP_TRY(hipStreamSynchronize(st));
+        for (size_t i = 0; i < items.size(); ++i)
+            std::memcpy(costs[a + i], s->host1.data() + items[i].off, (size_t)items[i].w * items[i].h * sizeof(float));
+    }
+    return C3D_OK;
+}
+
+int c3d_pm_filter(c3d_pm_session* s, const uint32_t* views, uint32_t n, const c3d_view_state* states, const c3d_depth_snapshot* snap,
+                  uint8_t* const* counts) {
+    if (!s || !snap || (n && (!views || !states || !counts))) return fail(C3D_INVALID_ARGUMENT, "null argument");
+    int rc = set_device(s->be);
+    if (rc) return rc;
+    rc = ensure_snapshot(s, snap);
+    if (rc) return rc;
+    const uint32_t level = snap->level;
+    rc = check_views(s, level, views, n, states);
+    if (rc) return rc;
+    hipStream_t st = s->be->stream;
+    for (auto [a, b] : batches(s, views, n, level)) {
+        std::vector<BatchItem> items;
+        long long npix;
+        int max_nsrc;
+        rc = set_batch(s, views, a, b, level, items, npix, max_nsrc);
+        if (rc) return rc;
+        pack(s, states + a, items, npix, false);
+        HIP_TRY(hipMemcpyAsync(s->plane.p, s->host4.data(), (size_t)npix * sizeof(float4), hipMemcpyHostToDevice, st));
+        KParams k = params_for(s, level, true, items, max_nsrc);
+        k.ntiles = pix_tiles(items.data(), (int)items.size());
+        HIP_TRY(s->ks->filter(k, st));
+        std::vector<uint8_t> hc((size_t)npix);
+        HIP_TRY(hipMemcpyAsync(hc.data(), s->count.p, (size_t)npix, hipMemcpyDeviceToHost, st));
+        HIP_TRY(hipStreamSynchronize(st));
+        for (size_t i = 0; i < items.size(); ++i)
+            std::memcpy(counts[a + i], hc.data() + items[i].off, (size_t)items[i].w * items[i].h);
+    }
+    return C3D_OK;
+}
+
+int c3d_pm_upsample(c3d_pm_session* s, uint32_t level, const uint32_t* views, uint32_t n, const c3d_view_state* low, float sigma_s,
+                    float sigma_c, c3d_view_state* out) {
+    if (!s || (n && (!views || !low || !out))) return fail(C3D_INVALID_ARGUMENT, "null argument");
+    if (level == 0) return fail(C3D_INVALID_ARGUMENT, "upsampling needs level >= 1");
+    int rc = set_device(s->be);
+    if (rc) return rc;
+    rc = check_views(s, level - 1, views, n, low);
+    if (rc) return rc;
+    rc = check_views(s, level, views, n, out);
+    if (rc) return rc;
+    hipStream_t st = s->be->stream;
+    for (auto [a, b] : batches(s, views, n, level)) {
+        std::vector<BatchItem> items;
+        long long npix;
+        int max_nsrc;
+        rc = set_batch(s, views, a, b, level, items, npix, max_nsrc);
+        if (rc) return rc;
+        // low-resolution planes into the prior buffer
+        std::vector<BatchItem> low_items;
+        long long lpix = 0;
+        for (uint32_t i = a; i < b; ++i) {
+            BatchItem li{};
+            li.off = lpix;
+            li.w = (int)low[i].width;
+            li.h = (int)low[i].height;
+            low_items.push_back(li);
+            items[i - a].low_off = (int)lpix;
+            lpix += (long long)li.w * li.h;
+        }
+        HIP_TRY(hipMemcpyAsync(s->d_batch.p, items.data(), items.size() * sizeof(BatchItem), hipMemcpyHostToDevice, st));
+        HIP_TRY(s->prior.reserve(lpix));
+        pack(s, low + a, low_items, lpix, false);
+        HIP_TRY(hipMemcpyAsync(s->prior.p, s->host4.data(), (size_t)lpix * sizeof(float4), hipMemcpyHostToDevice, st));
+        KParams k = params_for(s, level, false, items, max_nsrc);
+        k.up_inv2s = 1.0f / (2.0f * sigma_s * sigma_s);
+        k.up_inv2c = 1.0f / (2.0f * sigma_c * sigma_c);
+        k.ntiles = pix_tiles(items.data(), (int)items.size());
+        HIP_TRY(launch_upsample(k, st));
+        s->host4.resize((size_t)npix);
+        HIP_TRY(hipMemcpyAsync(s->host4.data(), s->plane.p, (size_t)npix * sizeof(float4), hipMemcpyDeviceToHost, st));
+        HIP_TRY(hipStreamSynchronize(st));
+        unpack(s->host4, items, out + a);
+    }
+    return C3D_OK;
+}
+
+int c3d_pm_median(c3d_pm_session* s, uint32_t level, const uint32_t* views, uint32_t n, const c3d_view_state* in,
+                  c3d_view_state* out) {
+    if (!s || (n && (!views || !in || !out))) return fail(C3D_INVALID_ARGUMENT, "null argument");
+    int rc = set_device(s->be);
+    if (rc) return rc;
+    rc = check_views(s, level, views, n, in);
+    if (rc) return rc;
+    rc = check_views(s, level, views, n, out);
+    if (rc) return rc;
+    hipStream_t st = s->be->stream;
+    for (auto [a, b] : batches(s, views, n, level)) {
+        std::vector<BatchItem> items;
+        long long npix;
+        int max_nsrc;
+        rc = set_batch(s, views, a, b, level, items, npix, max_nsrc);
+        if (rc) return rc;
+        HIP_TRY(s->prior.reserve(npix));
+        pack(s, in + a, items, npix, false);
+        HIP_TRY(hipMemcpyAsync(s->prior.p, s->host4.data(), (size_t)npix * sizeof(float4), hipMemcpyHostToDevice, st));
+        KParams k = params_for(s, level, false, items, max_nsrc);
+        k.ntiles = pix_tiles(items.data(), (int)items.size());
+        HIP_TRY(launch_median(k, st));
+        s->host4.resize((size_t)npix);
+        HIP_TRY(hipMemcpyAsync(s->host4.data(), s->plane.p, (size_t)npix * sizeof(float4), hipMemcpyDeviceToHost, st));
+        HIP_TRY(hipStreamSynchronize(st));
+        unpack(s->host4, items, out + a);
+        for (size_t i = 0; i < items.size(); ++i)
+            if (out[a + i].cost && in[a + i].cost && out[a + i].cost != in[a + i].cost)
+                std::memcpy(out[a + i].cost, in[a + i].cost, (size_t)items[i].w * items[i].h * sizeof(float));
+    }
+    return C3D_OK;
+}
+
+int c3d_pm_densify(c3d_pm_session* s, const uint32_t* views, uint32_t n, const c3d_schedule* sch, c3d_view_state* out,
+                   uint8_t* const* counts, float* const* raw_depth, c3d_timing* timing) {
+    if (!s || !sch || !views || n == 0) return fail(C3D_INVALID_ARGUMENT, "null argument");
+    if (s->nl != 1) return fail(C3D_INVALID_ARGUMENT, "c3d_pm_densify handles single-level inputs");
+    int rc = set_device(s->be);
+    if (rc) return rc;
+    rc = check_views(s, 0, views, n, out);
+    if (rc) return rc;
+    for (uint32_t i = 0; i < n; ++i)
+        for (uint32_t j = 0; j < i; ++j)
+            if (views[i] == views[j]) return fail(C3D_INVALID_ARGUMENT, "duplicate view in c3d_pm_densify");
+    hipStream_t st = s->be->stream;
+    std::vector<BatchItem> items;
+    long long npix;
+    int max_nsrc;
+    rc = set_batch(s, views, 0, n, 0, items, npix, max_nsrc);
+    if (rc) return rc;
+    HIP_TRY(s->snap.reserve(npix));
+    HIP_TRY(s->plane2.reserve(npix));
+    HIP_TRY(s->need.reserve(n));
+    // snapshot offsets: every view of the batch at its batch offset, others absent
+    std::vector<long long> offs((size_t)s->nv, -1);
+    for (auto& it : items) offs[it.view] = it.off;
+    HIP_TRY(hipMemcpyAsync(s->snap_off.p, offs.data(), offs.size() * sizeof(long long), hipMemcpyHostToDevice, st));
+    s->snap_valid = false;  // the snapshot buffer now holds device-made maps
+    std::vector<uint8_t> need(n);
+    for (uint32_t i = 0; i < n; ++i) {
+        uint32_t nd = std::min<uint32_t>(s->filter_copy.min_num_consistent, (uint32_t)s->vd[views[i]].nsrc);
+        need[i] = (uint8_t)std::min<uint32_t>(std::max<uint32_t>(nd, 1u), 255u);
+    }
+    HIP_TRY(hipMemcpyAsync(s->need.p, need.data(), n, hipMemcpyHostToDevice, st));
+    const bool time_halves = timing != nullptr;
+    s->pool_used = 0;
+    HIP_TRY(hipEventRecord(s->ev[0], st));
+    uint32_t run_id = sch->first_run_id;
+    KParams k = params_for(s, 0, false, items, max_nsrc);
+    k.run_id = (int)run_id++;
+    k.random_init = 1;
+    rc = launch_run(s, k, items, s->pm_copy, false, sch->photometric_iters, time_halves);
+    if (rc) return rc;
+    HIP_TRY(hipEventRecord(s->ev[1], st));
+    for (uint32_t r = 0; r < sch->geometric_rounds; ++r) {
+        HIP_TRY(launch_extract_depth(s->plane.p, s->snap.p, npix, st));
+        KParams g = params_for(s, 0, true, items, max_nsrc);
+        g.run_id = (int)run_id++;
+        g.random_init = 0;
+        rc = launch_run(s, g, items, s->pm_copy, true, sch->geometric_iters, time_halves);
+        if (rc) return rc;
+    }
+    HIP_TRY(hipEventRecord(s->ev[2], st));
+    if (s->filter_copy.median_filter) {
+        HIP_TRY(s->prior.reserve(npix));
+        HIP_TRY(hipMemcpyAsync(s->prior.p, s->plane.p, (size_t)npix * sizeof(float4), hipMemcpyDeviceToDevice, st));
+        KParams m = params_for(s, 0, false, items, max_nsrc);
+        m.ntiles = pix_tiles(items.data(), (int)items.size());
+        HIP_TRY(launch_median(m, st));
+    }
+    HIP_TRY(launch_extract_depth(s->plane.p, s->snap.p, npix, st));
+    KParams f = params_for(s, 0, true, items, max_nsrc);
+    f.ntiles = pix_tiles(items.data(), (int)items.size());
+    HIP_TRY(s->ks->filter(f, st));
+    HIP_TRY(launch_threshold(f, s->need.p, s->plane2.p, st));
+    HIP_TRY(hipEventRecord(s->ev[3], st));
+    HIP_TRY(hipEventSynchronize(s->ev[3]));
+    if (timing) {
+        float a = 0, b = 0, c = 0, d = 0;
+        HIP_TRY(hipEventElapsedTime(&a, s->ev[0], s->ev[1]));
+        HIP_TRY(hipEventElapsedTime(&b, s->ev[1], s->ev[2]));
+        HIP_TRY(hipEventElapsedTime(&c, s->ev[2], s->ev[3]));
+        HIP_TRY(hipEventElapsedTime(&d, s->ev[0], s->ev[3]));
+        timing->photometric_ms = a;
+        timing->geometric_ms = b;
+        timing->filter_ms = c;
+        timing->total_ms = d;
+        rc = pool_sum(s, &timing->half_kernel_ms, &timing->half_launches);
+        if (rc) return rc;
+    }
+    if (out) {
+        s->host4.resize((size_t)npix);
+        s->host1.resize((size_t)npix);
+        HIP_TRY(hipMemcpyAsync(s->host4.data(), s->plane2.p, (size_t)npix * sizeof(float4), hipMemcpyDeviceToHost, st));
+        HIP_TRY(hipMemcpyAsync(s->host1.data(), s->cost.p, (size_t)npix * sizeof(float), hipMemcpyDeviceToHost, st));
+        HIP_TRY(hipStreamSynchronize(st));
+        unpack(s->host4, items, out);
+        for (size_t i = 0; i < items.size(); ++i)
+            if (out[i].cost) std::memcpy(out[i].cost, s->host1.data() + items[i].off, (size_t)items[i].w * items[i].h * sizeof(float));
+    }
+    if (raw_depth) {
+        std::vector<float> hd((size_t)npix);
+        HIP_TRY(launch_extract_depth(s->plane.p, s->snap.p, npix, st));
+        HIP_TRY(hipMemcpyAsync(hd.data(), s->snap.p, (size_t)npix * sizeof(float), hipMemcpyDeviceToHost, st));
+        HIP_TRY(hipStreamSynchronize(st));
+        for (size_t i = 0; i < items.size(); ++i)
+            if (raw_depth[i]) std::memcpy(raw_depth[i], hd.data() + items[i].off, (size_t)items[i].w * items[i].h * sizeof(float));
+    }
+    if (counts) {
+        std::vector<uint8_t> hc((size_t)npix);
+        HIP_TRY(hipMemcpyAsync(hc.data(), s->count.p, (size_t)npix, hipMemcpyDeviceToHost, st));
+        HIP_TRY(hipStreamSynchronize(st));
+        for (size_t i = 0; i < items.size(); ++i)
+            if (counts[i]) std::memcpy(counts[i], hc.data() + items[i].off, (size_t)items[i].w * items[i].h);
+    }
+    return C3D_OK;
+}
+
+int c3d_pm_pair_costs(c3d_pm_session* s, uint32_t level, uint32_t view, uint32_t slot, uint32_t nq, const int32_t* xy,
+                      const float* planes, float* costs) {
+    if (!s || (nq && (!xy || !planes || !costs))) return fail(C3D_INVALID_ARGUMENT, "null argument");
+    int rc = set_device(s->be);
+    if (rc) return rc;
+    if ((int)level >= s->nl || (int)view >= s->nv) return fail(C3D_INVALID_ARGUMENT, "level or view out of range");
+    if ((int)slot >= s->vd[view].nsrc) return fail(C3D_INVALID_ARGUMENT, "source slot out of range");
+    const LevelDesc& R = s->lv[view * s->nl + level];
+    for (uint32_t i = 0; i < nq; ++i)
+        if (xy[2 * i] < 0 || xy[2 * i + 1] < 0 || xy[2 * i] >= R.w || xy[2 * i + 1] >= R.h)
+            return fail(C3D_INVALID_ARGUMENT, "query pixel outside the image");
+    if (nq == 0) return C3D_OK;
+    hipStream_t st = s->be->stream;
+    HIP_TRY(s->q_xy.reserve(2 * (size_t)nq));
+    HIP_TRY(s->q_planes.reserve(nq));
+    HIP_TRY(s->q_out.reserve(nq));
+    HIP_TRY(hipMemcpyAsync(s->q_xy.p, xy, 2 * (size_t)nq * sizeof(int), hipMemcpyHostToDevice, st));
+    HIP_TRY(hipMemcpyAsync(s->q_planes.p, planes, (size_t)nq * sizeof(float4), hipMemcpyHostToDevice, st));
+    std::vector<BatchItem> none;
+    KParams k = params_for(s, level, false, none, 1);
+    HIP_TRY(s->ks->pair_costs(k, (int)view, (int)slot, (int)nq, s->q_xy.p, s->q_planes.p, s->q_out.p, st));
+    HIP_TRY(hipMemcpyAsync(costs, s->q_out.p, (size_t)nq * sizeof(float), hipMemcpyDeviceToHost, st));
+    HIP_TRY(hipStreamSynchronize(st));
+    return C3D_OK;
+}
+
+}  // extern "C"

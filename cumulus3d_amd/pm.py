@@ -1,0 +1,217 @@
+"""Python binding of the gfx950 PatchStereo engine (include/c3d_pm.h) with the reference's interface.
+
+``HipPatchMatch`` stands in for ``CudaPatchMatch`` behind ``PatchMatchBackend``
+(crates/cuda/src/patchmatch.rs:285-298, crates/dense/src/kernel.rs:239-245) and ``Session`` for
+``PatchMatchSession`` (kernel.rs:191-237): ``run``, ``evaluate``, ``filter``, ``upsample``,
+``median_filter`` with the same argument meaning. Errors raise :class:`PmError` carrying the
+status the Rust shim maps to ``Error::InvalidArgument`` / ``Error::Invariant`` (patchmatch.rs:234-236).
+
+There is no CPU fallback: without the built library or without a gfx950 device every call raises.
+"""
+from __future__ import annotations
+
+import ctypes as C
+import os
+import threading
+from typing import List, Optional, Sequence
+
+import numpy as np
+
+from . import abi
+
+_LIB_DIR = os.path.join(os.path.dirname(os.path.abspath(__file__)), "lib")
+_libs = {}
+_lock = threading.Lock()
+
+EXPORTS = [
+    "c3d_pm_create", "c3d_pm_name", "c3d_pm_destroy", "c3d_pm_is_available", "c3d_pm_begin", "c3d_pm_run",
+    "c3d_pm_evaluate", "c3d_pm_filter", "c3d_pm_upsample", "c3d_pm_median", "c3d_pm_end", "c3d_pm_last_error",
+    "c3d_pm_densify", "c3d_pm_pair_costs",
+]
+
+
+class PmError(RuntimeError):
+    """A failed engine call: ``status`` is the c3d_status code, the message the engine's text."""
+
+    def __init__(self, status: int, message: str):
+        super().__init__(f"[{status}] {message}")
+        self.status = status
+        self.message = message
+
+
+def library_path(repro: bool = False) -> str:
+    return os.path.join(_LIB_DIR, "libc3d_pm_repro.so" if repro else "libc3d_pm.so")
+
+
+def load(repro: bool = False) -> C.CDLL:
+    """Loads the engine library (built by cumulus3d_amd/build.py); raises if it is missing."""
+    with _lock:
+        if repro in _libs:
+            return _libs[repro]
+        path = library_path(repro)
+        if not os.path.exists(path):
+            raise PmError(abi.C3D_NOT_AVAILABLE, f"{path} is missing: build the extension first (__graft_entry__.build())")
+        L = C.CDLL(path)
+        u32p = C.POINTER(C.c_uint32)
+        vsp = C.POINTER(abi.c3d_view_state)
+        snp = C.POINTER(abi.c3d_depth_snapshot)
+        L.c3d_pm_create.argtypes = [C.c_int, C.POINTER(abi.c3d_pm_options), C.POINTER(C.c_void_p)]
+        L.c3d_pm_name.argtypes = [C.c_void_p]
+        L.c3d_pm_name.restype = C.c_char_p
+        L.c3d_pm_destroy.argtypes = [C.c_void_p]
+        L.c3d_pm_destroy.restype = None
+        L.c3d_pm_is_available.argtypes = []
+        L.c3d_pm_begin.argtypes = [C.c_void_p, C.POINTER(abi.c3d_kernel_input), C.POINTER(C.c_void_p)]
+        L.c3d_pm_run.argtypes = [C.c_void_p, u32p, C.c_uint32, vsp, C.POINTER(abi.c3d_run_params), snp]
+        L.c3d_pm_evaluate.argtypes = [C.c_void_p, C.c_uint32, u32p, C.c_uint32, vsp, C.c_int, snp,
+                                      C.POINTER(C.POINTER(C.c_float))]
+        L.c3d_pm_filter.argtypes = [C.c_void_p, u32p, C.c_uint32, vsp, snp, C.POINTER(C.POINTER(C.c_uint8))]
+        L.c3d_pm_upsample.argtypes = [C.c_void_p, C.c_uint32, u32p, C.c_uint32, vsp, C.c_float, C.c_float, vsp]
+        L.c3d_pm_median.argtypes = [C.c_void_p, C.c_uint32, u32p, C.c_uint32, vsp, vsp]
+        L.c3d_pm_end.argtypes = [C.c_void_p]
+        L.c3d_pm_end.restype = None
+        L.c3d_pm_last_error.restype = C.c_char_p
+        L.c3d_pm_densify.argtypes = [C.c_void_p, u32p, C.c_uint32, C.POINTER(abi.c3d_schedule), vsp,
+                                     C.POINTER(C.POINTER(C.c_uint8)), C.POINTER(C.POINTER(C.c_float)),
+                                     C.POINTER(abi.c3d_timing)]
+        L.c3d_pm_pair_costs.argtypes = [C.c_void_p, C.c_uint32, C.c_uint32, C.c_uint32, C.c_uint32,
+                                        C.POINTER(C.c_int32), C.POINTER(C.c_float), C.POINTER(C.c_float)]
+        for n in ("c3d_pm_create", "c3d_pm_begin", "c3d_pm_run", "c3d_pm_evaluate", "c3d_pm_filter", "c3d_pm_upsample",
+                  "c3d_pm_median", "c3d_pm_densify", "c3d_pm_pair_costs", "c3d_pm_is_available"):
+            getattr(L, n).restype = C.c_int
+        _libs[repro] = L
+        return L
+
+
+def _check(L, rc: int):
+    if rc != 0:
+        raise PmError(rc, L.c3d_pm_last_error().decode(errors="replace"))
+
+
+def is_available(repro: bool = False) -> bool:
+    """is_available (crates/cuda/src/device.rs:80-89): a gfx950 device can be opened."""
+    return bool(load(repro).c3d_pm_is_available())
+
+
+class HipPatchMatch:
+    """PatchMatchBackend over device `ordinal` (CudaPatchMatch::new / try_default, patchmatch.rs:239-246)."""
+
+    def __init__(self, ordinal: int = 0, options: Optional[abi.c3d_pm_options] = None, repro: bool = False):
+        self.L = load(repro)
+        h = C.c_void_p()
+        _check(self.L, self.L.c3d_pm_create(ordinal, C.byref(options) if options is not None else None, C.byref(h)))
+        self.h = h
+
+    def name(self) -> str:
+        return self.L.c3d_pm_name(self.h).decode()
+
+    def begin(self, inp: abi.KernelInput) -> "Session":
+        return Session(self, inp)
+
+    def close(self):
+        if self.h:
+            self.L.c3d_pm_destroy(self.h)
+            self.h = None
+
+    def __del__(self):
+        try:
+            self.close()
+        except Exception:
+            pass
+
+
+class Session:
+    """PatchMatchSession (kernel.rs:191-237) over the engine; images are uploaded at creation."""
+
+    def __init__(self, backend: HipPatchMatch, inp: abi.KernelInput):
+        self.be = backend
+        self.L = backend.L
+        self.inp = inp
+        h = C.c_void_p()
+        _check(self.L, self.L.c3d_pm_begin(backend.h, C.byref(inp.c), C.byref(h)))
+        self.h = h
+
+    def close(self):
+        if self.h:
+            self.L.c3d_pm_end(self.h)
+            self.h = None
+
+    def __enter__(self):
+        return self
+
+    def __exit__(self, *a):
+        self.close()
+
+    def __del__(self):
+        try:
+            self.close()
+        except Exception:
+            pass
+
+    def run(self, views: Sequence[int], states: Sequence[abi.ViewState], params: abi.c3d_run_params,
+            snapshot: Optional[abi.DepthSnapshot] = None) -> None:
+        arr = abi.states_array(states)
+        _check(self.L, self.L.c3d_pm_run(self.h, abi.views_array(views), len(views), arr, C.byref(params),
+                                         C.byref(snapshot.c) if snapshot else None))
+
+    def evaluate(self, level: int, views: Sequence[int], states: Sequence[abi.ViewState], geometric: bool = False,
+                 snapshot: Optional[abi.DepthSnapshot] = None) -> List[np.ndarray]:
+        outs = [np.zeros((s.height, s.width), np.float32) for s in states]
+        _check(self.L, self.L.c3d_pm_evaluate(self.h, level, abi.views_array(views), len(views), abi.states_array(states),
+                                              int(geometric), C.byref(snapshot.c) if snapshot else None,
+                                              abi.ptr_array(outs, C.c_float)))
+        return outs
+
+    def filter(self, views: Sequence[int], states: Sequence[abi.ViewState], snapshot: abi.DepthSnapshot) -> List[np.ndarray]:
+        outs = [np.zeros((s.height, s.width), np.uint8) for s in states]
+        _check(self.L, self.L.c3d_pm_filter(self.h, abi.views_array(views), len(views), abi.states_array(states),
+                                            C.byref(snapshot.c), abi.ptr_array(outs, C.c_uint8)))
+        return outs
+
+    def upsample(self, level: int, views: Sequence[int], low: Sequence[abi.ViewState], sigma_s: float,
+                 sigma_c: float) -> List[abi.ViewState]:
+        outs = []
+        for v in views:
+            h, w = self.inp.level_shape(v, level)
+            outs.append(abi.ViewState(w, h))
+        _check(self.L, self.L.c3d_pm_upsample(self.h, level, abi.views_array(views), len(views), abi.states_array(low),
+                                              sigma_s, sigma_c, abi.states_array(outs)))
+        return outs
+
+    def median_filter(self, level: int, views: Sequence[int], states: Sequence[abi.ViewState]) -> List[abi.ViewState]:
+        outs = [abi.ViewState(s.width, s.height) for s in states]
+        _check(self.L, self.L.c3d_pm_median(self.h, level, abi.views_array(views), len(views), abi.states_array(states),
+                                            abi.states_array(outs)))
+        for o, s in zip(outs, states):
+            o.cost[...] = s.cost
+        return outs
+
+    def densify(self, views: Sequence[int], photometric_iters: int, geometric_rounds: int, geometric_iters: int,
+                first_run_id: int = 1, download: bool = True):
+        """Device-resident single-level schedule (densify.rs:350-454). Returns (states with thresholded
+        depth, counts, raw depth, timing) or (None, None, None, timing) without download."""
+        sch = abi.c3d_schedule(photometric_iters, geometric_rounds, geometric_iters, first_run_id)
+        tm = abi.c3d_timing()
+        if not download:
+            _check(self.L, self.L.c3d_pm_densify(self.h, abi.views_array(views), len(views), C.byref(sch), None, None, None,
+                                                 C.byref(tm)))
+            return None, None, None, tm
+        outs, counts, raws = [], [], []
+        for v in views:
+            h, w = self.inp.level_shape(v, 0)
+            outs.append(abi.ViewState(w, h))
+            counts.append(np.zeros((h, w), np.uint8))
+            raws.append(np.zeros((h, w), np.float32))
+        _check(self.L, self.L.c3d_pm_densify(self.h, abi.views_array(views), len(views), C.byref(sch), abi.states_array(outs),
+                                             abi.ptr_array(counts, C.c_uint8), abi.ptr_array(raws, C.c_float), C.byref(tm)))
+        return outs, counts, raws, tm
+
+    def pair_costs(self, level: int, view: int, slot: int, xy: np.ndarray, planes: np.ndarray) -> np.ndarray:
+        """Per-hypothesis photometric cost of explicit planes (c3d_pm_pair_costs)."""
+        xy = np.ascontiguousarray(xy, np.int32).reshape(-1, 2)
+        planes = np.ascontiguousarray(planes, np.float32).reshape(-1, 4)
+        out = np.zeros(len(xy), np.float32)
+        _check(self.L, self.L.c3d_pm_pair_costs(self.h, level, view, slot, len(xy), xy.ctypes.data_as(C.POINTER(C.c_int32)),
+                                                planes.ctypes.data_as(C.POINTER(C.c_float)),
+                                                out.ctypes.data_as(C.POINTER(C.c_float))))
+        return out

@@ -1,0 +1,190 @@
+"""CPU oracle for the PatchStereo hot path — TEST INFRASTRUCTURE ONLY.
+
+Loaded only by tests/, __graft_entry__.smoke() and bench.py's cpu_baseline leg, as the checker
+(or the timed CPU baseline). The product (cumulus3d_amd) never imports this package.
+
+``Oracle`` wraps oracle/build/libpm_oracle.so (built from pm_oracle.c by oracle/Makefile), a C
+restatement of the reference's CUDA PatchStereo kernels and host rules; see pm_oracle.c for the
+function-by-function citations and what pins each part.
+"""
+from __future__ import annotations
+
+import ctypes as C
+import os
+import subprocess
+from typing import Optional, Sequence
+
+import numpy as np
+
+from cumulus3d_amd import abi
+
+_HERE = os.path.dirname(os.path.abspath(__file__))
+LIB_PATH = os.path.join(_HERE, "build", "libpm_oracle.so")
+_lib = None
+
+
+def build(force: bool = False) -> str:
+    """Compiles the oracle with its Makefile (gcc)."""
+    cmd = ["make", "-C", _HERE]
+    if force:
+        cmd.insert(1, "-B")
+    subprocess.run(cmd, check=True, stdout=subprocess.DEVNULL)
+    return LIB_PATH
+
+
+def lib():
+    global _lib
+    if _lib is None:
+        if not os.path.exists(LIB_PATH):
+            build()
+        L = C.CDLL(LIB_PATH)
+        u32p = C.POINTER(C.c_uint32)
+        vsp = C.POINTER(abi.c3d_view_state)
+        L.pmo_hash_bytes.argtypes = [C.c_char_p, C.c_size_t]
+        L.pmo_hash_bytes.restype = C.c_uint64
+        L.pmo_mix64.argtypes = [C.c_uint64]
+        L.pmo_mix64.restype = C.c_uint64
+        L.pmo_rng_state.argtypes = [C.c_uint64, C.c_uint64, C.c_uint32, C.c_uint32, C.c_uint32, C.c_uint32, C.c_uint64]
+        L.pmo_rng_state.restype = C.c_uint64
+        L.pmo_rng_next.argtypes = [C.POINTER(C.c_uint64)]
+        L.pmo_rng_next.restype = C.c_float
+        for n in ("pmo_erf", "pmo_emission_norm"):
+            getattr(L, n).argtypes = [C.c_double]
+            getattr(L, n).restype = C.c_double
+        for n in ("pmo_emission", "pmo_visibility_probability", "pmo_triangulation_prior", "pmo_incident_prior"):
+            getattr(L, n).argtypes = [C.c_double, C.c_double]
+            getattr(L, n).restype = C.c_double
+        L.pmo_resolution_prior.argtypes = [C.POINTER(C.c_double), C.c_double, C.c_double, C.c_double]
+        L.pmo_resolution_prior.restype = C.c_double
+        dp = C.POINTER(C.c_double)
+        L.pmo_plane_transfer.argtypes = [C.c_double, dp, dp, dp, dp]
+        L.pmo_plane_transfer.restype = C.c_int
+        L.pmo_quantile_sorted.argtypes = [dp, C.c_size_t, C.c_double]
+        L.pmo_quantile_sorted.restype = C.c_double
+        L.pmo_bilinear_clamped.argtypes = [C.POINTER(C.c_float), C.c_size_t, C.c_size_t, C.c_float, C.c_float]
+        L.pmo_bilinear_clamped.restype = C.c_float
+        L.pmo_set_rcp_mode.argtypes = [C.c_int]
+        L.pmo_expf.argtypes = [C.c_float]
+        L.pmo_expf.restype = C.c_float
+        L.pmo_sincosf.argtypes = [C.c_float, C.POINTER(C.c_float), C.POINTER(C.c_float)]
+        fp = C.POINTER(C.c_float)
+        L.pmo_level_pair.argtypes = [fp, fp, fp, fp, fp, fp]
+        L.pmo_num_levels.argtypes = [C.c_uint32, C.c_uint32, C.c_uint32]
+        L.pmo_num_levels.restype = C.c_uint32
+        L.pmo_build_pyramid.argtypes = [C.POINTER(C.c_uint8), C.c_uint32, C.c_uint32, dp, C.c_uint32,
+                                        C.POINTER(C.POINTER(C.c_uint8)), fp]
+        L.pmo_create.argtypes = [C.POINTER(abi.c3d_kernel_input), C.c_int]
+        L.pmo_create.restype = C.c_void_p
+        L.pmo_destroy.argtypes = [C.c_void_p]
+        L.pmo_last_error.restype = C.c_char_p
+        L.pmo_run.argtypes = [C.c_void_p, u32p, C.c_uint32, vsp, C.POINTER(abi.c3d_run_params),
+                              C.POINTER(abi.c3d_depth_snapshot)]
+        L.pmo_evaluate.argtypes = [C.c_void_p, C.c_uint32, u32p, C.c_uint32, vsp, C.c_int,
+                                   C.POINTER(abi.c3d_depth_snapshot), C.POINTER(C.POINTER(C.c_float))]
+        L.pmo_filter.argtypes = [C.c_void_p, u32p, C.c_uint32, vsp, C.POINTER(abi.c3d_depth_snapshot),
+                                 C.POINTER(C.POINTER(C.c_uint8))]
+        L.pmo_upsample.argtypes = [C.c_void_p, C.c_uint32, u32p, C.c_uint32, vsp, C.c_float, C.c_float, vsp]
+        L.pmo_median.argtypes = [C.c_void_p, C.c_uint32, u32p, C.c_uint32, vsp, vsp]
+        L.pmo_densify.argtypes = [C.c_void_p, u32p, C.c_uint32, C.POINTER(abi.c3d_schedule), vsp,
+                                  C.POINTER(C.POINTER(C.c_uint8)), C.POINTER(C.POINTER(C.c_float))]
+        L.pmo_run_rows.argtypes = [C.c_void_p, C.c_uint32, vsp, C.POINTER(abi.c3d_run_params), C.c_uint32, C.c_uint32]
+        L.pmo_ncc_cost_f32.argtypes = [C.c_void_p, C.c_uint32, C.c_uint32, C.c_uint32, C.c_int, C.c_int, fp]
+        L.pmo_ncc_cost_f32.restype = C.c_float
+        L.pmo_ncc_cost_f64.argtypes = [C.c_void_p, C.c_uint32, C.c_uint32, C.c_uint32, C.c_int, C.c_int, dp]
+        L.pmo_ncc_cost_f64.restype = C.c_double
+        for n in ("pmo_run", "pmo_evaluate", "pmo_filter", "pmo_upsample", "pmo_median", "pmo_densify", "pmo_run_rows"):
+            getattr(L, n).restype = C.c_int
+        _lib = L
+    return _lib
+
+
+class OracleError(RuntimeError):
+    pass
+
+
+def _check(rc: int):
+    if rc != 0:
+        raise OracleError(f"oracle error {rc}: {lib().pmo_last_error().decode()}")
+
+
+class Oracle:
+    """CPU restatement of a PatchStereo session over a KernelInput (same operations as the C ABI)."""
+
+    def __init__(self, inp: abi.KernelInput, threads: Optional[int] = None, rcp_mode: int = 0):
+        self.inp = inp
+        L = lib()
+        L.pmo_set_rcp_mode(rcp_mode)
+        self.threads = threads or os.cpu_count() or 1
+        self.ctx = L.pmo_create(C.byref(inp.c), self.threads)
+        if not self.ctx:
+            raise OracleError(L.pmo_last_error().decode())
+
+    def close(self):
+        if self.ctx:
+            lib().pmo_destroy(self.ctx)
+            self.ctx = None
+
+    def __del__(self):
+        self.close()
+
+    def run(self, views: Sequence[int], states: Sequence[abi.ViewState], params: abi.c3d_run_params,
+            snapshot: Optional[abi.DepthSnapshot] = None):
+        arr = abi.states_array(states)
+        _check(lib().pmo_run(self.ctx, abi.views_array(views), len(views), arr, C.byref(params),
+                             C.byref(snapshot.c) if snapshot else None))
+
+    def evaluate(self, level: int, views, states, geometric=False, snapshot=None):
+        outs = [np.zeros((s.height, s.width), np.float32) for s in states]
+        _check(lib().pmo_evaluate(self.ctx, level, abi.views_array(views), len(views), abi.states_array(states),
+                                  int(geometric), C.byref(snapshot.c) if snapshot else None,
+                                  abi.ptr_array(outs, C.c_float)))
+        return outs
+
+    def filter(self, views, states, snapshot):
+        outs = [np.zeros((s.height, s.width), np.uint8) for s in states]
+        _check(lib().pmo_filter(self.ctx, abi.views_array(views), len(views), abi.states_array(states),
+                                C.byref(snapshot.c), abi.ptr_array(outs, C.c_uint8)))
+        return outs
+
+    def upsample(self, level, views, low, sigma_s, sigma_c):
+        outs = []
+        for v in views:
+            h, w = self.inp.level_shape(v, level)
+            outs.append(abi.ViewState(w, h))
+        _check(lib().pmo_upsample(self.ctx, level, abi.views_array(views), len(views), abi.states_array(low),
+                                  sigma_s, sigma_c, abi.states_array(outs)))
+        return outs
+
+    def median_filter(self, level, views, states):
+        outs = [abi.ViewState(s.width, s.height) for s in states]
+        _check(lib().pmo_median(self.ctx, level, abi.views_array(views), len(views), abi.states_array(states),
+                                abi.states_array(outs)))
+        for o, s in zip(outs, states):
+            o.cost[...] = s.cost
+        return outs
+
+    def densify(self, views, photometric_iters, geometric_rounds, geometric_iters, first_run_id=1):
+        """Single-level schedule (densify.rs:350-454): returns (states with thresholded depth,
+        counts, raw depth)."""
+        sch = abi.c3d_schedule(photometric_iters, geometric_rounds, geometric_iters, first_run_id)
+        outs, counts, raws = [], [], []
+        for v in views:
+            h, w = self.inp.level_shape(v, 0)
+            outs.append(abi.ViewState(w, h))
+            counts.append(np.zeros((h, w), np.uint8))
+            raws.append(np.zeros((h, w), np.float32))
+        _check(lib().pmo_densify(self.ctx, abi.views_array(views), len(views), C.byref(sch), abi.states_array(outs),
+                                 abi.ptr_array(counts, C.c_uint8), abi.ptr_array(raws, C.c_float)))
+        return outs, counts, raws
+
+    def run_rows(self, view, state, params, row0, row1):
+        arr = abi.states_array([state])
+        _check(lib().pmo_run_rows(self.ctx, view, arr, C.byref(params), row0, row1))
+
+    def ncc_cost_f32(self, level, view, slot, x, y, plane):
+        p = (C.c_float * 4)(*[float(q) for q in plane])
+        return lib().pmo_ncc_cost_f32(self.ctx, level, view, slot, x, y, p)
+
+    def ncc_cost_f64(self, level, view, slot, x, y, plane):
+        p = (C.c_double * 4)(*[float(q) for q in plane])
+        return lib().pmo_ncc_cost_f64(self.ctx, level, view, slot, x, y, p)

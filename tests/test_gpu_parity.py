@@ -1,0 +1,265 @@
+"""GPU parity of the gfx950 engine against the CPU oracle, through the C ABI.
+
+Workload: config 1 of the reference's hot path — seeded fBm terrain (200 x 200 m, +-15 m relief,
+random albedo) imaged by 3 pinhole cameras 30 m up at 45 deg oblique, f = 500 px, 640 x 360 u8,
+each view using the other two as sources, window radius 5 step 2, 5 iterations, depth 20-80 m.
+
+Bars:
+  * per-hypothesis NCC cost within 1e-4 absolute of the FP64 evaluation (north_star tolerance);
+  * the reproducible build (C3D_REPRO_RCP, Newton reciprocal) equals the FP32 oracle bit for bit
+    for costs, runs, evaluation, read-out, upsampling and the median filter;
+  * the production build (v_rcp_f32) agrees with the oracle within 1 % relative depth on >= 99 %
+    of valid pixels, normals within 2 deg on >= 99 % (north_star tolerance).
+"""
+import numpy as np
+import pytest
+
+from cumulus3d_amd import abi, pm, scenes
+
+pytestmark = pytest.mark.gpu
+
+SRC3 = [[1, 2], [0, 2], [0, 1]]
+
+
+def _c1(width=640, height=360, seed=1):
+    rig = scenes.oblique_rig(3, baseline=6.0)
+    sc = scenes.make_terrain_scene(rig, width, height, focal=500.0 * width / 640.0, seed=seed)
+    pmp, fl = abi.fast_profile()
+    inp = scenes.kernel_input(sc, SRC3, [(20.0, 80.0)] * 3, pmp, fl)
+    return sc, inp
+
+
+@pytest.fixture(scope="module")
+def c1():
+    return _c1()
+
+
+@pytest.fixture(scope="module")
+def small():
+    return _c1(192, 108, seed=4)
+
+
+def _oracle(inp, rcp_mode):
+    import oracle
+
+    return oracle.Oracle(inp, threads=16, rcp_mode=rcp_mode)
+
+
+def _queries(sc, v, n, seed):
+    rng = np.random.default_rng(seed)
+    h, w = sc.views[v].gray.shape
+    xs = rng.integers(0, w, n)
+    ys = rng.integers(0, h, n)
+    planes = np.zeros((n, 4), np.float32)
+    for i, (x, y) in enumerate(zip(xs, ys)):
+        if i % 3 == 0:  # random hypothesis
+            nrm = rng.normal(size=3)
+            nrm[2] = -abs(nrm[2]) - 0.2
+            d = rng.uniform(20, 80)
+        else:  # near the true surface
+            nrm = scenes.np.asarray(sc.gt_normal[v][y, x], float) + rng.normal(0, 0.05, 3)
+            d = float(sc.gt_depth[v][y, x]) * rng.uniform(0.98, 1.02)
+        nrm = nrm / np.linalg.norm(nrm)
+        planes[i] = [*nrm, d]
+    return np.stack([xs, ys], 1).astype(np.int32), planes
+
+
+def test_backend_opens_gfx950():
+    assert pm.is_available()
+    be = pm.HipPatchMatch()
+    assert be.name().startswith("hip(")
+
+
+def test_pair_costs_match_fp64(c1):
+    sc, inp = c1
+    be = pm.HipPatchMatch()
+    o = _oracle(inp, 0)
+    with be.begin(inp) as s:
+        for v in range(3):
+            xy, planes = _queries(sc, v, 3000, seed=10 + v)
+            for slot in range(2):
+                got = s.pair_costs(0, v, slot, xy, planes)
+                ref = np.array([o.ncc_cost_f64(0, v, slot, int(x), int(y), p) for (x, y), p in zip(xy, planes.astype(float))])
+                both = (ref < 2.0) & (got < 2.0)
+                assert both.sum() > 1500
+                err = np.abs(got[both] - ref[both])
+                print(f"view {v} slot {slot}: {both.sum()} costs, max |gpu - fp64| = {err.max():.2e}")
+                assert err.max() < 1e-4
+                # decisions to return 2 (outside, behind, flat) agree except at exact thresholds
+                assert ((ref == 2.0) != (got == 2.0)).sum() <= 2
+
+
+def test_pair_costs_repro_bit_exact(c1):
+    sc, inp = c1
+    be = pm.HipPatchMatch(repro=True)
+    o = _oracle(inp, 1)
+    with be.begin(inp) as s:
+        xy, planes = _queries(sc, 0, 2000, seed=3)
+        got = s.pair_costs(0, 0, 1, xy, planes)
+        ref = np.array([o.ncc_cost_f32(0, 0, 1, int(x), int(y), p) for (x, y), p in zip(xy, planes)], np.float32)
+        assert np.array_equal(got.view(np.uint32), ref.view(np.uint32))
+
+
+def _states(inp, views, level=0):
+    out = []
+    for v in views:
+        h, w = inp.level_shape(v, level)
+        out.append(abi.ViewState(w, h))
+    return out
+
+
+def _agreement(d_gpu, d_ref, n_gpu, n_ref):
+    valid = (d_gpu > 0) & (d_ref > 0)
+    rel = np.abs(d_gpu[valid] - d_ref[valid]) / d_ref[valid]
+    cosang = np.clip((n_gpu[valid] * n_ref[valid]).sum(-1), -1, 1)
+    ang = np.degrees(np.arccos(cosang))
+    return valid.mean(), (rel <= 0.01).mean(), (ang <= 2.0).mean()
+
+
+def test_run_repro_bit_exact(c1):
+    """Random-init photometric run, 5 iterations (config 1): identical to the oracle."""
+    sc, inp = c1
+    be = pm.HipPatchMatch(repro=True)
+    o = _oracle(inp, 1)
+    p = abi.run_params(0, False, True, 5, 1, False)
+    g = _states(inp, [0, 1, 2])
+    r = _states(inp, [0, 1, 2])
+    with be.begin(inp) as s:
+        s.run([0, 1, 2], g, p)
+    o.run([0, 1, 2], r, p)
+    for a, b in zip(g, r):
+        assert np.array_equal(a.depth.view(np.uint32), b.depth.view(np.uint32))
+        assert np.array_equal(a.normal.view(np.uint32), b.normal.view(np.uint32))
+        assert np.array_equal(a.cost.view(np.uint32), b.cost.view(np.uint32))
+
+
+def test_run_production_within_tolerance(c1):
+    sc, inp = c1
+    be = pm.HipPatchMatch()
+    o = _oracle(inp, 0)
+    p = abi.run_params(0, False, True, 5, 1, False)
+    g = _states(inp, [0, 1, 2])
+    r = _states(inp, [0, 1, 2])
+    with be.begin(inp) as s:
+        s.run([0, 1, 2], g, p)
+    o.run([0, 1, 2], r, p)
+    for v, (a, b) in enumerate(zip(g, r)):
+        valid, dfrac, nfrac = _agreement(a.depth, b.depth, a.normal, b.normal)
+        print(f"view {v}: valid {valid:.4f} depth<=1% {dfrac:.5f} normal<=2deg {nfrac:.5f}")
+        assert dfrac >= 0.99 and nfrac >= 0.99
+
+
+def test_evaluate_filter_repro_bit_exact(small):
+    sc, inp = small
+    be = pm.HipPatchMatch(repro=True)
+    o = _oracle(inp, 1)
+    views = [0, 1, 2]
+    g = _states(inp, views)
+    p = abi.run_params(0, False, True, 3, 1, False)
+    o.run(views, g, p)
+    snap = abi.DepthSnapshot(7, 0, [s.depth for s in g])
+    with be.begin(inp) as s:
+        ce = s.evaluate(0, views, g, False)
+        cg = s.evaluate(0, views, g, True, snap)
+        cnt = s.filter(views, g, snap)
+        # geometric run from the same states
+        gg = [x.copy() for x in g]
+        s.run(views, gg, abi.run_params(0, True, False, 2, 2, False), snap)
+    oe = o.evaluate(0, views, g, False)
+    og = o.evaluate(0, views, g, True, snap)
+    oc = o.filter(views, g, snap)
+    og2 = [x.copy() for x in g]
+    o.run(views, og2, abi.run_params(0, True, False, 2, 2, False), snap)
+    for i in range(3):
+        assert np.array_equal(ce[i].view(np.uint32), oe[i].view(np.uint32))
+        assert np.array_equal(cg[i].view(np.uint32), og[i].view(np.uint32))
+        assert np.array_equal(cnt[i], oc[i])
+        assert np.array_equal(gg[i].depth.view(np.uint32), og2[i].depth.view(np.uint32))
+        assert np.array_equal(gg[i].normal.view(np.uint32), og2[i].normal.view(np.uint32))
+    assert sum(int(c.sum()) for c in cnt) > 0
+
+
+def test_densify_production_within_tolerance(small):
+    """Device-resident single-level schedule (photometric 5, one geometric round of 2, read-out)."""
+    sc, inp = small
+    be = pm.HipPatchMatch()
+    o = _oracle(inp, 0)
+    with be.begin(inp) as s:
+        go, gc, graw, tm = s.densify([0, 1, 2], 5, 1, 2)
+    ro, rc, rraw = o.densify([0, 1, 2], 5, 1, 2)
+    assert tm.total_ms > 0 and tm.half_launches == 2 * (5 + 2)
+    for v in range(3):
+        valid, dfrac, nfrac = _agreement(graw[v], rraw[v], go[v].normal, ro[v].normal)
+        print(f"view {v}: raw valid {valid:.4f} depth<=1% {dfrac:.5f}")
+        assert dfrac >= 0.99
+        # the read-out counts agree on nearly all pixels
+        assert (gc[v] == rc[v]).mean() > 0.98
+
+
+def test_densify_repro_matches_trait_calls(small):
+    """c3d_pm_densify equals the same schedule driven through run/filter (and the oracle) bit for bit."""
+    sc, inp = small
+    be = pm.HipPatchMatch(repro=True)
+    o = _oracle(inp, 1)
+    with be.begin(inp) as s:
+        go, gc, graw, tm = s.densify([0, 1, 2], 3, 1, 2)
+    ro, rc, rraw = o.densify([0, 1, 2], 3, 1, 2)
+    for v in range(3):
+        assert np.array_equal(graw[v].view(np.uint32), rraw[v].view(np.uint32))
+        assert np.array_equal(gc[v], rc[v])
+        assert np.array_equal(go[v].depth.view(np.uint32), ro[v].depth.view(np.uint32))
+
+
+def _two_level(seed=5):
+    import oracle
+
+    rig = scenes.oblique_rig(3, baseline=6.0)
+    sc = scenes.make_terrain_scene(rig, 160, 96, focal=125.0, seed=seed)
+    levels = []
+    L = oracle.lib()
+    import ctypes as C
+
+    for v in sc.views:
+        low = np.zeros((48, 80), np.uint8)
+        outs = (C.POINTER(C.c_uint8) * 1)(low.ctypes.data_as(C.POINTER(C.c_uint8)))
+        ko = (C.c_float * 8)()
+        L.pmo_build_pyramid(v.gray.ctypes.data_as(C.POINTER(C.c_uint8)), 160, 96, (C.c_double * 4)(*v.k), 2, outs, ko)
+        levels.append([abi.LevelData(low, list(ko[4:8])), abi.LevelData(v.gray, list(ko[0:4]))])
+    pmp, fl = abi.fast_profile()
+    return sc, scenes.kernel_input(sc, SRC3, [(20.0, 80.0)] * 3, pmp, fl, levels=levels)
+
+
+def test_upsample_median_repro_bit_exact():
+    sc, inp = _two_level()
+    be = pm.HipPatchMatch(repro=True)
+    o = _oracle(inp, 1)
+    views = [0, 1, 2]
+    low = _states(inp, views, 0)
+    o.run(views, low, abi.run_params(0, False, True, 3, 1, False))
+    with be.begin(inp) as s:
+        up = s.upsample(1, views, low, 1.0, 0.04)
+        med = s.median_filter(1, views, up)
+    oup = o.upsample(1, views, low, 1.0, 0.04)
+    omed = o.median_filter(1, views, oup)
+    for i in range(3):
+        assert np.array_equal(up[i].depth.view(np.uint32), oup[i].depth.view(np.uint32))
+        assert np.array_equal(up[i].normal.view(np.uint32), oup[i].normal.view(np.uint32))
+        assert np.array_equal(med[i].depth.view(np.uint32), omed[i].depth.view(np.uint32))
+    assert (up[0].depth > 0).mean() > 0.9
+
+
+def test_errors_map_to_invalid_argument(small):
+    sc, inp = small
+    be = pm.HipPatchMatch()
+    bad = abi.KernelInput(inp.views, abi.PmParams(window_radius=21), inp.filter)
+    with pytest.raises(pm.PmError) as e:
+        be.begin(bad)
+    assert e.value.status == abi.C3D_INVALID_ARGUMENT
+    with be.begin(inp) as s:
+        wrong = [abi.ViewState(10, 10)]
+        with pytest.raises(pm.PmError) as e:
+            s.run([0], wrong, abi.run_params(0, False, True, 1, 1, False))
+        assert e.value.status == abi.C3D_INVALID_ARGUMENT
+        with pytest.raises(pm.PmError) as e:
+            s.run([0], _states(inp, [0]), abi.run_params(0, True, False, 1, 1, False), None)
+        assert e.value.status == abi.C3D_INVALID_ARGUMENT
