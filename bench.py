@@ -46,6 +46,23 @@ def device_sync():
         pass
 
 
+def copy_calibration(mib):
+    """A plain device-to-device copy of `mib` MiB (hipMemcpy, run by the runtime's copy kernel), so
+    that the HBM counters of a profile can be scaled by the same counters on a copy of known size."""
+    hip = C.CDLL("libamdhip64.so")
+    n = mib << 20
+    a, b = C.c_void_p(), C.c_void_p()
+    if hip.hipMalloc(C.byref(a), C.c_size_t(n)) or hip.hipMalloc(C.byref(b), C.c_size_t(n)):
+        raise RuntimeError("hipMalloc failed")
+    hip.hipMemset(a, 1, C.c_size_t(n))
+    hip.hipDeviceSynchronize()
+    hip.hipMemcpy(b, a, C.c_size_t(n), 3)  # hipMemcpyDeviceToDevice
+    hip.hipDeviceSynchronize()
+    hip.hipFree(a)
+    hip.hipFree(b)
+    log(f"copy calibration: {mib} MiB device-to-device")
+
+
 def make_workload(width, height, nviews, noise, seed):
     rig = scenes.oblique_rig(nviews, baseline=6.0)
     sc = scenes.make_terrain_scene(rig, width, height, focal=500.0, seed=seed, noise_sigma=noise, noise_seed=seed + 1)
@@ -87,6 +104,8 @@ def main():
     ap.add_argument("--cpu-rows", type=int, default=320)
     ap.add_argument("--no-cpu", action="store_true")
     ap.add_argument("--repro", action="store_true", help="use the reproducible-reciprocal build")
+    ap.add_argument("--calibrate-copy", type=int, default=0, metavar="MIB",
+                    help="also run a device-to-device copy of this many MiB (profiling calibration)")
     args = ap.parse_args()
     if args.gpus != 1:
         print(f"bench.py measures one GPU only (every box here has one); --gpus {args.gpus} is not supported", file=sys.stderr)
@@ -95,6 +114,8 @@ def main():
         print("no gfx950 device: the engine has no CPU path", file=sys.stderr)
         sys.exit(1)
 
+    if args.calibrate_copy:
+        copy_calibration(args.calibrate_copy)
     t0 = time.perf_counter()
     sc, inp = make_workload(args.width, args.height, args.views, 0.01, seed=1)
     log(f"scene {args.views} x {args.width}x{args.height} rendered in {time.perf_counter() - t0:.1f} s")

@@ -316,13 +316,16 @@ __device__ __forceinline__ bool homography(const LevelDesc& R, const LevelPair& 
     return true;
 }
 
-// One source sample in gray levels through the quad table (see the layout note above).
-__device__ __forceinline__ float quad_sample(const uint32_t* q0, int qpitch, float w1, float h1, float u, float v) {
+// One source sample in gray levels through the quad table (see the layout note above). `qbase` is
+// the table's first entry (x0, y0) = (-2, -2); the entry index (y0 + 2) qpitch + (x0 + 2) is formed
+// exactly in FP32 (integers below 2^24) from the clamped floor of the sample position.
+__device__ __forceinline__ float quad_sample(const uint32_t* qbase, float qpf, float koff, float w1, float h1, float u, float v) {
     float xf = floorf(u), yf = floorf(v);
     float ax = u - xf, ay = v - yf;
-    int xi = (int)fminf(fmaxf(xf, -2.0f), w1);
-    int yi = (int)fminf(fmaxf(yf, -2.0f), h1);
-    uint32_t q = q0[yi * qpitch + xi];
+    float xc = __builtin_amdgcn_fmed3f(xf, -2.0f, w1);
+    float yc = __builtin_amdgcn_fmed3f(yf, -2.0f, h1);
+    uint32_t i = (uint32_t)fmaf(yc, qpf, xc + koff);
+    uint32_t q = qbase[i];
     float p00 = (float)(q & 255u), p10 = (float)((q >> 8) & 255u);
     float p01 = (float)((q >> 16) & 255u), p11 = (float)(q >> 24);
     float top = fmaf(ax, p10 - p00, p00);
@@ -334,13 +337,13 @@ __device__ __forceinline__ float quad_sample(const uint32_t* q0, int qpitch, flo
 // sample (only needed when a window corner fails it; z is monotone in each offset, so the corners
 // bound every sample).
 template <int WR, int WSTEP, bool CHECK>
-__device__ __forceinline__ void window_sums(const RefWin<WR, WSTEP>& rw, const uint32_t* q0, int qpitch, float w1, float h1,
-                                            const float* H, float& Ss, float& SSs, float& IS) {
+__device__ __forceinline__ void window_sums(const RefWin<WR, WSTEP>& rw, const uint32_t* qbase, float qpf, float koff, float w1,
+                                            float h1, const float* H, float& Ss, float& SSs, float& IS) {
     constexpr int WN = Win<WR, WSTEP>::WN;
     Ss = 0.f;
     SSs = 0.f;
     IS = 0.f;
-#pragma unroll
+#pragma unroll(CHECK ? 1 : WN)
     for (int a = 0; a < WN; ++a) {
         const float oy = (float)(-WR + a * WSTEP);
         const float rz = fmaf(H[7], oy, H[8]);
@@ -354,7 +357,7 @@ __device__ __forceinline__ void window_sums(const RefWin<WR, WSTEP>& rw, const u
             float sv = 0.0f;
             if (!CHECK || zz > 1e-8f) {
                 float r = rcp_sample(zz);
-                sv = quad_sample(q0, qpitch, w1, h1, fmaf(H[0], ox, ru) * r, fmaf(H[3], ox, rv) * r);
+                sv = quad_sample(qbase, qpf, koff, w1, h1, fmaf(H[0], ox, ru) * r, fmaf(H[3], ox, rv) * r);
             }
             float sp = sv - rw.ic;
             float ws = rw.w[k] * sp;
@@ -377,15 +380,16 @@ __device__ __forceinline__ float ncc_cost(const KParams& P, const RefWin<WR, WST
     float v = H[5] * iz;
     if (!(u >= 0.f && v >= 0.f && u <= (float)(S.w - 1) && v <= (float)(S.h - 1))) return 2.0f;
     if (rw.var < P.vmin255) return 2.0f;
-    const uint32_t* q0 = P.quad + S.qoff;
+    const uint32_t* qbase = P.quad + (S.qoff - 2LL * S.qpitch - 2);
+    const float qpf = (float)S.qpitch, koff = 2.0f * qpf + 2.0f;
     const float w1 = (float)S.w, h1 = (float)S.h;
     constexpr float lo = (float)(-WR), hi = (float)Win<WR, WSTEP>::LAST;
     const float za = fmaf(H[7], lo, H[8]), zb = fmaf(H[7], hi, H[8]);
     const bool all_front = fmaf(H[6], lo, za) > 1e-8f && fmaf(H[6], hi, za) > 1e-8f && fmaf(H[6], lo, zb) > 1e-8f &&
                            fmaf(H[6], hi, zb) > 1e-8f;
     float Ss, SSs, IS;
-    if (all_front) window_sums<WR, WSTEP, false>(rw, q0, S.qpitch, w1, h1, H, Ss, SSs, IS);
-    else window_sums<WR, WSTEP, true>(rw, q0, S.qpitch, w1, h1, H, Ss, SSs, IS);
+    if (all_front) window_sums<WR, WSTEP, false>(rw, qbase, qpf, koff, w1, h1, H, Ss, SSs, IS);
+    else window_sums<WR, WSTEP, true>(rw, qbase, qpf, koff, w1, h1, H, Ss, SSs, IS);
     float iW = 1.0f / rw.W;
     float mus = Ss * iW;
     float vars = fmaf(-mus, mus, SSs * iW);

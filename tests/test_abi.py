@@ -1,0 +1,87 @@
+"""The C ABI boundary on the CPU: the engine library loads, exports every entry point that
+include/c3d_pm.h declares, its POD layouts match the ctypes mirror (as a C compiler lays them out),
+and without a gfx950 device it reports NOT_AVAILABLE instead of falling back to anything."""
+import ctypes as C
+import os
+import re
+import subprocess
+import tempfile
+
+import pytest
+
+from cumulus3d_amd import abi, pm
+
+ROOT = os.path.dirname(os.path.dirname(os.path.abspath(__file__)))
+HEADER = os.path.join(ROOT, "include", "c3d_pm.h")
+
+
+def declared_functions():
+    text = open(HEADER).read()
+    text = re.sub(r"/\*.*?\*/", "", text, flags=re.S)
+    return sorted(set(re.findall(r"\b(c3d_pm_\w+)\s*\(", text)))
+
+
+def test_header_declares_the_trait_surface():
+    names = declared_functions()
+    for n in ["c3d_pm_create", "c3d_pm_name", "c3d_pm_begin", "c3d_pm_run", "c3d_pm_evaluate", "c3d_pm_filter",
+              "c3d_pm_upsample", "c3d_pm_median", "c3d_pm_end", "c3d_pm_destroy", "c3d_pm_last_error"]:
+        assert n in names
+
+
+@pytest.mark.parametrize("repro", [False, True])
+def test_library_exports_every_declared_symbol(repro):
+    L = C.CDLL(pm.library_path(repro))
+    missing = [n for n in declared_functions() if not hasattr(L, n)]
+    assert not missing, missing
+    assert sorted(pm.EXPORTS) == declared_functions()
+
+
+STRUCTS = {
+    "c3d_pm_options": abi.c3d_pm_options,
+    "c3d_level_image": abi.c3d_level_image,
+    "c3d_pair_geometry": abi.c3d_pair_geometry,
+    "c3d_kernel_view": abi.c3d_kernel_view,
+    "c3d_pm_params": abi.c3d_pm_params,
+    "c3d_filter_params": abi.c3d_filter_params,
+    "c3d_kernel_input": abi.c3d_kernel_input,
+    "c3d_view_state": abi.c3d_view_state,
+    "c3d_depth_snapshot": abi.c3d_depth_snapshot,
+    "c3d_run_params": abi.c3d_run_params,
+    "c3d_schedule": abi.c3d_schedule,
+    "c3d_timing": abi.c3d_timing,
+}
+
+
+def test_struct_layouts_match_the_header():
+    lines = ['#include <stdio.h>', '#include <stddef.h>', f'#include "{HEADER}"', "int main(void) {"]
+    for name, cls in STRUCTS.items():
+        lines.append(f'printf("{name} size %zu\\n", sizeof({name}));')
+        for fname, _ in cls._fields_:
+            lines.append(f'printf("{name}.{fname} %zu\\n", offsetof({name}, {fname}));')
+    lines.append("return 0; }")
+    with tempfile.TemporaryDirectory() as d:
+        src, exe = os.path.join(d, "l.c"), os.path.join(d, "l")
+        open(src, "w").write("\n".join(lines))
+        subprocess.run(["gcc", "-std=c11", src, "-o", exe], check=True)
+        out = subprocess.run([exe], check=True, stdout=subprocess.PIPE, text=True).stdout.split("\n")
+    got = dict(l.rsplit(" ", 1) for l in out if l)
+    for name, cls in STRUCTS.items():
+        assert int(got[f"{name} size"]) == C.sizeof(cls), name
+        for fname, _ in cls._fields_:
+            assert int(got[f"{name}.{fname}"]) == getattr(cls, fname).offset, (name, fname)
+
+
+def test_no_device_means_not_available():
+    if pm.is_available():
+        pytest.skip("a gfx950 device is present")
+    with pytest.raises(pm.PmError) as e:
+        pm.HipPatchMatch(0)
+    assert e.value.status == abi.C3D_NOT_AVAILABLE
+
+
+def test_missing_library_fails_loudly(monkeypatch, tmp_path):
+    monkeypatch.setattr(pm, "_LIB_DIR", str(tmp_path))
+    monkeypatch.setattr(pm, "_libs", {})
+    with pytest.raises(pm.PmError) as e:
+        pm.load()
+    assert e.value.status == abi.C3D_NOT_AVAILABLE
