@@ -103,14 +103,13 @@ def main():
     ap.add_argument("--geometric-iters", type=int, default=2)
     ap.add_argument("--cpu-rows", type=int, default=320)
     ap.add_argument("--no-cpu", action="store_true")
-    ap.add_argument("--repro", action="store_true", help="use the reproducible-reciprocal build")
     ap.add_argument("--calibrate-copy", type=int, default=0, metavar="MIB",
                     help="also run a device-to-device copy of this many MiB (profiling calibration)")
     args = ap.parse_args()
     if args.gpus != 1:
         print(f"bench.py measures one GPU only (every box here has one); --gpus {args.gpus} is not supported", file=sys.stderr)
         sys.exit(2)
-    if not pm.is_available(args.repro):
+    if not pm.is_available():
         print("no gfx950 device: the engine has no CPU path", file=sys.stderr)
         sys.exit(1)
 
@@ -121,7 +120,7 @@ def main():
     log(f"scene {args.views} x {args.width}x{args.height} rendered in {time.perf_counter() - t0:.1f} s")
     views = list(range(args.views))
     npix = args.width * args.height
-    be = pm.HipPatchMatch(0, repro=args.repro)
+    be = pm.HipPatchMatch(0)
     sess = be.begin(inp)
     log(f"session open on {be.name()}")
     for i in range(args.warmup):
@@ -183,7 +182,6 @@ def main():
             "photometric_iters": args.photometric_iters,
             "geometric_rounds": args.geometric_rounds,
             "geometric_iters": args.geometric_iters,
-            "library": "repro" if args.repro else "production",
         },
         "roofline": {
             "bound": "hbm",

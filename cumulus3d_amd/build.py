@@ -1,9 +1,6 @@
 """Builds the native libraries of cumulus3d_amd in-tree (cumulus3d_amd/lib/):
 
 * libc3d_pm.so        the gfx950 PatchStereo engine (HIP kernels + C ABI of include/c3d_pm.h)
-* libc3d_pm_repro.so  the same sources with C3D_REPRO_RCP: the per-sample projective reciprocal is
-                      a Newton reciprocal reproducible on the CPU (used to check the kernels against
-                      the CPU oracle bit for bit)
 * libc3d_scenes.so    the synthetic terrain renderer used by tests and benchmarks (host C)
 
 All HIP code is compiled for gfx950 only, with -ffp-contract=off (every multiply-add is written out)
@@ -68,9 +65,11 @@ def build_scenes() -> str:
 
 def build_all() -> list[str]:
     os.makedirs(LIB, exist_ok=True)
-    with ThreadPoolExecutor(3) as ex:
-        futs = [ex.submit(_engine, "c3d_pm", []), ex.submit(_engine, "c3d_pm_repro", ["-DC3D_REPRO_RCP"]),
-                ex.submit(build_scenes)]
+    stale = os.path.join(LIB, "libc3d_pm_repro.so")
+    if os.path.exists(stale):
+        os.remove(stale)
+    with ThreadPoolExecutor(2) as ex:
+        futs = [ex.submit(_engine, "c3d_pm", []), ex.submit(build_scenes)]
         return [f.result() for f in futs]
 
 

@@ -11,10 +11,9 @@
 //          (ViewState packed as in patchmatch.rs:327-348); cost f32, best view u8, count u8 alike.
 //
 // Arithmetic: every operation is written out (explicit fmaf where an FMA is intended; the library
-// is compiled with -ffp-contract=off) in the order the CPU oracle uses, IEEE division and square
-// root, and the polynomial exp / sin-cos below, so that the device and the oracle agree bit for
-// bit except for the per-sample reciprocal of the projective divide, which is v_rcp_f32 unless the
-// library is built with C3D_REPRO_RCP (then a Newton reciprocal the oracle reproduces).
+// is compiled with -ffp-contract=off) in the order the CPU oracle uses, with IEEE division and
+// square root, the polynomial exp / sin-cos below and a Newton reciprocal for the per-sample
+// projective divide, so that the device and the CPU oracle agree bit for bit.
 #pragma once
 
 #include <hip/hip_runtime.h>
@@ -138,13 +137,13 @@ __device__ __forceinline__ unsigned long long rng_state(unsigned long long seed,
 
 // ---------------------------------------------------------------- elementary functions
 
-// exp: Cody-Waite reduction, degree-7 Taylor polynomial, scale by 2^n; 0 below -86.
+// exp: Cody-Waite reduction, degree-7 Taylor polynomial, scale by 2^n; 0 below -86, +inf above
+// 88, NaN kept. Branch-free: the polynomial runs on the argument clamped to [-86, 88] and the
+// special values are selected afterwards (same results as testing first).
 __device__ __forceinline__ float c3d_expf(float x) {
-    if (x != x) return x;
-    if (!(x > -86.0f)) return 0.0f;
-    if (x > 88.0f) return __builtin_inff();
-    float n = rintf(x * 1.44269504088896341f);
-    float r = fmaf(-n, 0.693145751953125f, x);
+    const float xc = fminf(fmaxf(x, -86.0f), 88.0f);
+    float n = rintf(xc * 1.44269504088896341f);
+    float r = fmaf(-n, 0.693145751953125f, xc);
     r = fmaf(-n, 1.42860677e-06f, r);
     float p = 1.98412698e-04f;
     p = fmaf(p, r, 1.38888889e-03f);
@@ -155,7 +154,10 @@ __device__ __forceinline__ float c3d_expf(float x) {
     p = fmaf(p, r, 1.0f);
     p = fmaf(p, r, 1.0f);
     int ni = (int)n;
-    return p * __int_as_float((ni + 127) << 23);
+    float e = p * __int_as_float((ni + 127) << 23);
+    e = x > -86.0f ? e : 0.0f;
+    e = x > 88.0f ? __builtin_inff() : e;
+    return x != x ? x : e;
 }
 
 // sin and cos: quadrant reduction, Taylor polynomials on |r| <= pi/4.
@@ -183,9 +185,10 @@ __device__ __forceinline__ void c3d_sincosf(float x, float& so, float& co) {
     else { so = -c; co = s; }
 }
 
-// Reciprocal of the projective divide of one source sample (z > 1e-8).
+// Reciprocal of the projective divide of one source sample: bit-pattern seed and three Newton
+// steps (accurate to about 1 ulp for positive normal z, reproducible on any IEEE machine; costs
+// under 2 % of the kernel against v_rcp_f32).
 __device__ __forceinline__ float rcp_sample(float z) {
-#ifdef C3D_REPRO_RCP
     float r = __uint_as_float(0x7EF311C3u - __float_as_uint(z));
     float e = fmaf(-z, r, 1.0f);
     r = fmaf(r, e, r);
@@ -194,9 +197,6 @@ __device__ __forceinline__ float rcp_sample(float z) {
     e = fmaf(-z, r, 1.0f);
     r = fmaf(r, e, r);
     return r;
-#else
-    return __builtin_amdgcn_rcpf(z);
-#endif
 }
 
 // ---------------------------------------------------------------- vectors
@@ -235,7 +235,7 @@ struct Win {
 // weighted deviations wd_k = w_k (I_k - mu) in gray levels, two-pass variance.
 template <int WR, int WSTEP>
 struct RefWin {
-    float ic, W, var;
+    float ic, W, iW, var;
     float w[Win<WR, WSTEP>::WN2];
     float wd[Win<WR, WSTEP>::WN2];
 };
@@ -260,7 +260,6 @@ struct RefTile {
 template <int WR, int WSTEP, class REF>
 __device__ __forceinline__ void ref_stats(const KParams& P, const REF& ref, int x, int y, RefWin<WR, WSTEP>& rw) {
     constexpr int WN = Win<WR, WSTEP>::WN;
-    float ik[WN * WN];
     rw.ic = ref.at(x, y);
     float W = 0.0f, S = 0.0f;
 #pragma unroll
@@ -274,7 +273,6 @@ __device__ __forceinline__ void ref_stats(const KParams& P, const REF& ref, int 
             float d = rw.ic - v;
             float sp = (float)(dx * dx + dy * dy) * P.inv2s;
             float w = c3d_expf(-fmaf(d * d, P.inv2c255, sp));
-            ik[k] = v;
             rw.w[k] = w;
             W += w;
             S = fmaf(w, v, S);
@@ -283,13 +281,20 @@ __device__ __forceinline__ void ref_stats(const KParams& P, const REF& ref, int 
     float mu = S / W;
     float V = 0.0f;
 #pragma unroll
-    for (int k = 0; k < WN * WN; ++k) {
-        float dk = ik[k] - mu;
-        float wd = rw.w[k] * dk;
-        rw.wd[k] = wd;
-        V = fmaf(wd, dk, V);
+    for (int a = 0; a < WN; ++a) {
+        const int dy = -WR + a * WSTEP;
+#pragma unroll
+        for (int b = 0; b < WN; ++b) {
+            const int dx = -WR + b * WSTEP;
+            const int k = a * WN + b;
+            float dk = ref.at(x + dx, y + dy) - mu;
+            float wd = rw.w[k] * dk;
+            rw.wd[k] = wd;
+            V = fmaf(wd, dk, V);
+        }
     }
     rw.W = W;
+    rw.iW = 1.0f / W;
     rw.var = V / W;
 }
 
@@ -299,8 +304,8 @@ __device__ __forceinline__ void ref_stats(const KParams& P, const REF& ref, int 
 // (dx, dy, 1) of pixel (x, y), H = (A + b n^T K^-1 / delta) T(x, y). Its third column is
 // A (x, y, 1)^T + b / d exactly (n^T K^-1 (x, y, 1)^T / delta = 1 / d), free of the cancellation of
 // the untranslated form for strongly slanted planes.
-__device__ __forceinline__ bool homography(const LevelDesc& R, const LevelPair& L, float x, float y, float d, f3 n, float* H) {
-    float ifx = 1.0f / R.fx, ify = 1.0f / R.fy;
+__device__ __forceinline__ bool homography(const LevelDesc& R, const LevelPair& L, float ifx, float ify, float x, float y, float d,
+                                           f3 n, float* H) {
     f3 ray = mk3((x - R.cx) * ifx, (y - R.cy) * ify, 1.0f);
     float delta = d * dot3(n, ray);
     if (!(delta < -1e-12f)) return false;
@@ -317,15 +322,16 @@ __device__ __forceinline__ bool homography(const LevelDesc& R, const LevelPair& 
 }
 
 // One source sample in gray levels through the quad table (see the layout note above). `qbase` is
-// the table's first entry (x0, y0) = (-2, -2); the entry index (y0 + 2) qpitch + (x0 + 2) is formed
-// exactly in FP32 (integers below 2^24) from the clamped floor of the sample position.
-__device__ __forceinline__ float quad_sample(const uint32_t* qbase, float qpf, float koff, float w1, float h1, float u, float v) {
+// the table's first entry (x0, y0) = (-2, -2); the byte offset 4 ((y0 + 2) qpitch + (x0 + 2)) is formed
+// exactly in FP32 (multiples of 4 below 2^26) from the clamped floor of the sample position, and the
+// load uses a uniform base plus a 32-bit offset.
+__device__ __forceinline__ float quad_sample(const uint32_t* qbase, float qp4, float koff4, float w1, float h1, float u, float v) {
     float xf = floorf(u), yf = floorf(v);
     float ax = u - xf, ay = v - yf;
     float xc = __builtin_amdgcn_fmed3f(xf, -2.0f, w1);
     float yc = __builtin_amdgcn_fmed3f(yf, -2.0f, h1);
-    uint32_t i = (uint32_t)fmaf(yc, qpf, xc + koff);
-    uint32_t q = qbase[i];
+    uint32_t off = (uint32_t)fmaf(yc, qp4, fmaf(xc, 4.0f, koff4));
+    uint32_t q = *(const uint32_t*)((const char*)qbase + off);
     float p00 = (float)(q & 255u), p10 = (float)((q >> 8) & 255u);
     float p01 = (float)((q >> 16) & 255u), p11 = (float)(q >> 24);
     float top = fmaf(ax, p10 - p00, p00);
@@ -333,17 +339,19 @@ __device__ __forceinline__ float quad_sample(const uint32_t* qbase, float qpf, f
     return fmaf(ay, bot - top, top);
 }
 
-// Window sums of one hypothesis x source (H translated to the pixel). CHECK: test z > 1e-8 per
-// sample (only needed when a window corner fails it; z is monotone in each offset, so the corners
-// bound every sample).
+// Window sums of one hypothesis x source (H translated to the pixel). A sample whose projective
+// depth z is not above 1e-8 contributes the value 0 (patchmatch.cu:271-276). CHECK = false when all
+// four window corners have z > 1e-8: z is monotone in each offset (every step is a correctly
+// rounded affine operation), so then every sample does. With CHECK the sample is computed anyway
+// (any position is clamped into the quad table) and replaced by a select.
 template <int WR, int WSTEP, bool CHECK>
-__device__ __forceinline__ void window_sums(const RefWin<WR, WSTEP>& rw, const uint32_t* qbase, float qpf, float koff, float w1,
+__device__ __forceinline__ void window_sums(const RefWin<WR, WSTEP>& rw, const uint32_t* qbase, float qp4, float koff4, float w1,
                                             float h1, const float* H, float& Ss, float& SSs, float& IS) {
     constexpr int WN = Win<WR, WSTEP>::WN;
     Ss = 0.f;
     SSs = 0.f;
     IS = 0.f;
-#pragma unroll(CHECK ? 1 : WN)
+#pragma unroll
     for (int a = 0; a < WN; ++a) {
         const float oy = (float)(-WR + a * WSTEP);
         const float rz = fmaf(H[7], oy, H[8]);
@@ -354,11 +362,9 @@ __device__ __forceinline__ void window_sums(const RefWin<WR, WSTEP>& rw, const u
             const int k = a * WN + b;
             const float ox = (float)(-WR + b * WSTEP);
             const float zz = fmaf(H[6], ox, rz);
-            float sv = 0.0f;
-            if (!CHECK || zz > 1e-8f) {
-                float r = rcp_sample(zz);
-                sv = quad_sample(qbase, qpf, koff, w1, h1, fmaf(H[0], ox, ru) * r, fmaf(H[3], ox, rv) * r);
-            }
+            const float r = rcp_sample(zz);
+            float sv = quad_sample(qbase, qp4, koff4, w1, h1, fmaf(H[0], ox, ru) * r, fmaf(H[3], ox, rv) * r);
+            if (CHECK) sv = zz > 1e-8f ? sv : 0.0f;
             float sp = sv - rw.ic;
             float ws = rw.w[k] * sp;
             Ss += ws;
@@ -381,16 +387,16 @@ __device__ __forceinline__ float ncc_cost(const KParams& P, const RefWin<WR, WST
     if (!(u >= 0.f && v >= 0.f && u <= (float)(S.w - 1) && v <= (float)(S.h - 1))) return 2.0f;
     if (rw.var < P.vmin255) return 2.0f;
     const uint32_t* qbase = P.quad + (S.qoff - 2LL * S.qpitch - 2);
-    const float qpf = (float)S.qpitch, koff = 2.0f * qpf + 2.0f;
+    const float qp4 = 4.0f * (float)S.qpitch, koff4 = 2.0f * qp4 + 8.0f;
     const float w1 = (float)S.w, h1 = (float)S.h;
     constexpr float lo = (float)(-WR), hi = (float)Win<WR, WSTEP>::LAST;
     const float za = fmaf(H[7], lo, H[8]), zb = fmaf(H[7], hi, H[8]);
     const bool all_front = fmaf(H[6], lo, za) > 1e-8f && fmaf(H[6], hi, za) > 1e-8f && fmaf(H[6], lo, zb) > 1e-8f &&
                            fmaf(H[6], hi, zb) > 1e-8f;
     float Ss, SSs, IS;
-    if (all_front) window_sums<WR, WSTEP, false>(rw, qbase, qpf, koff, w1, h1, H, Ss, SSs, IS);
-    else window_sums<WR, WSTEP, true>(rw, qbase, qpf, koff, w1, h1, H, Ss, SSs, IS);
-    float iW = 1.0f / rw.W;
+    if (all_front) window_sums<WR, WSTEP, false>(rw, qbase, qp4, koff4, w1, h1, H, Ss, SSs, IS);
+    else window_sums<WR, WSTEP, true>(rw, qbase, qp4, koff4, w1, h1, H, Ss, SSs, IS);
+    const float iW = rw.iW;
     float mus = Ss * iW;
     float vars = fmaf(-mus, mus, SSs * iW);
     if (vars < P.vmin255) return 2.0f;
@@ -430,7 +436,7 @@ __device__ __forceinline__ float pair_cost(const KParams& P, const RefWin<WR, WS
                                            const PairRec& pr, const LevelPair& lp, int x, int y, float4 h, float* geo) {
     float H[9];
     float c = 2.0f;
-    if (homography(R, lp, (float)x, (float)y, h.w, xyz(h), H)) c = ncc_cost<WR, WSTEP>(P, rw, S, H, x, y);
+    if (homography(R, lp, 1.0f / R.fx, 1.0f / R.fy, (float)x, (float)y, h.w, xyz(h), H)) c = ncc_cost<WR, WSTEP>(P, rw, S, H, x, y);
     if (P.geometric) *geo = fminf(geo_error(P, R, S, pr, x, y, h.w), P.gmax);
     return c;
 }

@@ -41,34 +41,28 @@ __device__ __forceinline__ float4 pick4(const float4 (&a)[N], int i) {
     return r;
 }
 
-// Weighted sum of one hypothesis' costs over the sources with positive weight (refinement and
-// coarse-prior candidates, patchmatch.cu:608-652).
-template <int WR, int WSTEP>
-__device__ __forceinline__ float candidate_cost(const KParams& P, const RefWin<WR, WSTEP>& rw, const LevelDesc& R, int view, int nsrc,
-                                                const float* wv, int x, int y, float4 h, float den) {
-    if (!(den > 0.f)) return 2.f;
-    float nm = 0.f;
-    const PairRec* prs = P.pairs + (size_t)view * MAXSRC;
-    const LevelPair* lps = P.lpairs + ((size_t)view * P.nlevels + P.level) * MAXSRC;
-#pragma nounroll
-    for (int j = 0; j < nsrc; ++j) {
-        float wj = wv[j * P.wv_stride];
-        if (!(wj > 0.f)) continue;
-        const PairRec& pr = prs[j];
-        const LevelDesc S = P.lv[pr.src * P.nlevels + P.level];
-        float g = 0.f;
-        float m = pair_cost<WR, WSTEP>(P, rw, R, S, pr, lps[j], x, y, h, &g);
-        nm += wj * (m + P.lambda * g);
-    }
-    return nm / den;
+// (photometric cost, truncated geometric error) of one hypothesis x one source with the pixel's
+// reciprocal focal lengths hoisted (patchmatch.cu:316-324)
+template <int WR, int WSTEP, bool GEO>
+__device__ __forceinline__ float pair_cost_h(const KParams& P, const RefWin<WR, WSTEP>& rw, const LevelDesc& R, const LevelDesc& S,
+                                             const PairRec& pr, const LevelPair& lp, float ifx, float ify, int x, int y, float4 h,
+                                             float& geo) {
+    float H[9];
+    float c = 2.0f;
+    if (homography(R, lp, ifx, ify, (float)x, (float)y, h.w, xyz(h), H)) c = ncc_cost<WR, WSTEP>(P, rw, S, H, x, y);
+    geo = GEO ? fminf(geo_error(P, R, S, pr, x, y, h.w), P.gmax) : 0.f;
+    return c;
 }
 
 // ================================================================ half step (patchmatch.cu:439-657)
-template <int WR, int WSTEP>
-__global__ __launch_bounds__(256) void k_half(KParams P) {
+// GEO: geometric-consistency run (the geometric term is compiled in only there).
+template <int WR, int WSTEP, bool GEO>
+__global__ __launch_bounds__(256) __attribute__((amdgpu_waves_per_eu(2, 2))) void k_half(KParams P) {
     constexpr int TW = HALF_SPAN + 2 * WR;
     constexpr int TH = HALF_ROWS + 2 * WR;
     __shared__ float tile[TW * TH];
+    // per-thread hypothesis table: the 9 candidates of step 2, then the refinement candidates
+    __shared__ float4 hyp_lds[9 * 256];
     extern __shared__ float wv_lds[];
 
     BatchItem it;
@@ -101,11 +95,14 @@ __global__ __launch_bounds__(256) void k_half(KParams P) {
     float4* pl = P.plane + base;
     const float* co = P.cost + base;
     const f3 ray = pixel_ray(R, x, y);
+    const float ifx = 1.0f / R.fx, ify = 1.0f / R.fy;
     float* wv = wv_lds + threadIdx.x;
+    float4* hyp = hyp_lds + threadIdx.x;  // hyp[i * 256]
 
-    // 1. candidates: current + best of 8 regions (patchmatch.cu:477-511)
-    float4 hyp[9];
-    hyp[0] = pl[pix];
+    // 1. candidates: current + best of 8 regions (patchmatch.cu:477-511). The 72 neighbour costs
+    //    are loaded unconditionally (out-of-image offsets read the pixel's own cost and are ignored).
+    const float4 hyp0 = pl[pix];
+    hyp[0] = hyp0;
 #pragma unroll
     for (int r = 0; r < 8; ++r) {
         const int dir = r & 3;
@@ -119,17 +116,18 @@ __global__ __launch_bounds__(256) void k_half(KParams P) {
             int dx, dy;
             if (strip) rot_dir(dir, 0, -(3 + 2 * k), dx, dy);
             else rot_dir(dir, V_OFF[k][0], V_OFF[k][1], dx, dy);
-            int qx = x + dx, qy = y + dy;
-            if (qx < 0 || qy < 0 || qx >= w || qy >= h) continue;
-            float cq = co[qy * w + qx];
-            if (cq < best) {
+            const int qx = x + dx, qy = y + dy;
+            const bool inb = qx >= 0 && qy >= 0 && qx < w && qy < h;
+            const int qi = inb ? qy * w + qx : pix;
+            const float cq = co[qi];
+            if (inb && cq < best) {
                 best = cq;
-                bq = qy * w + qx;
+                bq = qi;
                 bx = qx;
                 by = qy;
             }
         }
-        hyp[1 + r] = hyp[0];
+        float4 hr = hyp0;
         if (bq >= 0) {
             float4 q = pl[bq];
             f3 n = xyz(q);
@@ -137,9 +135,10 @@ __global__ __launch_bounds__(256) void k_half(KParams P) {
             float den = dot3(n, ray);
             if (den < -1e-6f && q.w > 0.f) {
                 float d = q.w * dot3(n, rq) / den;
-                if (d > 0.f && isfinite(d)) hyp[1 + r] = mk4(n, d);
+                if (d > 0.f && isfinite(d)) hr = mk4(n, d);
             }
         }
+        hyp[(1 + r) * 256] = hr;
     }
 
     // 2. view selection and aggregation (patchmatch.cu:513-594)
@@ -159,19 +158,16 @@ __global__ __launch_bounds__(256) void k_half(KParams P) {
         float prior;
         {
             float H[9];
-            bool ok = homography(R, lp, (float)x, (float)y, hyp[0].w, xyz(hyp[0]), H);
-            prior = view_prior<WR>(P, R, pr, H, ok, x, y, hyp[0]);
+            bool ok = homography(R, lp, ifx, ify, (float)x, (float)y, hyp0.w, xyz(hyp0), H);
+            prior = view_prior<WR>(P, R, pr, H, ok, x, y, hyp0);
         }
         wv[j * P.wv_stride] = 0.f;
         if (!(prior > 0.f)) continue;
         float m[9], gg[9];
 #pragma nounroll
         for (int i = 0; i < 9; ++i) {
-            float4 hi = pick4(hyp, i);
-            float H[9];
-            bool ok = homography(R, lp, (float)x, (float)y, hi.w, xyz(hi), H);
-            float mi = ok ? ncc_cost<WR, WSTEP>(P, rw, S, H, x, y) : 2.f;
-            float gi = P.geometric ? fminf(geo_error(P, R, S, pr, x, y, hi.w), P.gmax) : 0.f;
+            float gi;
+            const float mi = pair_cost_h<WR, WSTEP, GEO>(P, rw, R, S, pr, lp, ifx, ify, x, y, hyp[i * 256], gi);
 #pragma unroll
             for (int q = 0; q < 9; ++q)
                 if (q == i) {
@@ -227,7 +223,7 @@ __global__ __launch_bounds__(256) void k_half(KParams P) {
 #pragma unroll
         for (int i = 0; i < 9; ++i) agg[i] = 2.f;
     }
-    float4 best = pick4(hyp, bi9);
+    float4 best = hyp[bi9 * 256];
     float bcost = 2.f;
 #pragma unroll
     for (int q = 0; q < 9; ++q)
@@ -243,20 +239,39 @@ __global__ __launch_bounds__(256) void k_half(KParams P) {
     if (!(best.w > 0.f)) best.w = d_r;
     f3 n0 = xyz(best);
     f3 n_p = perturb_normal(st, n0, ray, P.phi);
-    float4 cand[7] = {mk4(n0, d_r), mk4(n_r, best.w), mk4(n_r, d_r), mk4(n0, d_p), mk4(n_p, best.w), mk4(n_p, d_p),
-                      make_float4(0.f, 0.f, 0.f, 0.f)};
+    float4* cand = hyp;  // the hypothesis table is free again
+    cand[0] = mk4(n0, d_r);
+    cand[256] = mk4(n_r, best.w);
+    cand[2 * 256] = mk4(n_r, d_r);
+    cand[3 * 256] = mk4(n0, d_p);
+    cand[4 * 256] = mk4(n_p, best.w);
+    cand[5 * 256] = mk4(n_p, d_p);
     int ncand = 6;
     if (P.use_prior && rw.var < P.weak_var255) {
         float4 q = P.prior[base + pix];
         if (q.w > 0.f && dot3(xyz(q), ray) < 0.f) {
-            cand[6] = q;
+            cand[6 * 256] = q;
             ncand = 7;
         }
     }
 #pragma nounroll
     for (int c = 0; c < ncand; ++c) {
-        float4 hc = pick4(cand, c);
-        float cc = candidate_cost<WR, WSTEP>(P, rw, R, it.view, nsrc, wv, x, y, hc, den);
+        const float4 hc = cand[c * 256];
+        float cc = 2.f;
+        if (den > 0.f) {
+            float nm = 0.f;
+#pragma nounroll
+            for (int j = 0; j < nsrc; ++j) {
+                float wj = wv[j * P.wv_stride];
+                if (!(wj > 0.f)) continue;
+                const PairRec& pr = prs[j];
+                const LevelDesc S = P.lv[pr.src * P.nlevels + P.level];
+                float g;
+                float m = pair_cost_h<WR, WSTEP, GEO>(P, rw, R, S, pr, lps[j], ifx, ify, x, y, hc, g);
+                nm += wj * (m + P.lambda * g);
+            }
+            cc = nm / den;
+        }
         if (cc < bcost) {
             bcost = cc;
             best = hc;
@@ -360,7 +375,7 @@ __global__ __launch_bounds__(256) void k_filter(KParams P) {
             float cphi = dot3(Sv, xyz(h)) / ls;
             if (cphi <= 0.f) continue;
             float H[9];
-            if (!homography(R, lps[j], (float)x, (float)y, h.w, xyz(h), H)) continue;
+            if (!homography(R, lps[j], 1.0f / R.fx, 1.0f / R.fy, (float)x, (float)y, h.w, xyz(h), H)) continue;
             float m = ncc_cost<WR, WSTEP>(P, rw, S, H, x, y);
             float e = P.ncc_norm * c3d_expf(-m * m * P.inv2ncc);
             if (e / (e + 0.5f) < P.q_min) continue;
@@ -388,7 +403,7 @@ __global__ __launch_bounds__(256) void k_pair_costs(KParams P, int view, int slo
     float4 h = planes[i];
     float H[9];
     float c = 2.f;
-    if (homography(R, lp, (float)x, (float)y, h.w, xyz(h), H)) c = ncc_cost<WR, WSTEP>(P, rw, S, H, x, y);
+    if (homography(R, lp, 1.0f / R.fx, 1.0f / R.fy, (float)x, (float)y, h.w, xyz(h), H)) c = ncc_cost<WR, WSTEP>(P, rw, S, H, x, y);
     out[i] = c;
 }
 
@@ -553,7 +568,8 @@ int pix_tiles(const BatchItem* items, int n) {
 template <int WR, int WSTEP>
 static hipError_t launch_half(const KParams& P, hipStream_t s) {
     size_t lds = (size_t)P.wv_stride * (P.max_nsrc > 0 ? P.max_nsrc : 1) * sizeof(float);
-    hipLaunchKernelGGL((k_half<WR, WSTEP>), dim3(P.ntiles), dim3(256), lds, s, P);
+    if (P.geometric) hipLaunchKernelGGL((k_half<WR, WSTEP, true>), dim3(P.ntiles), dim3(256), lds, s, P);
+    else hipLaunchKernelGGL((k_half<WR, WSTEP, false>), dim3(P.ntiles), dim3(256), lds, s, P);
     return hipGetLastError();
 }
 template <int WR, int WSTEP>
@@ -574,7 +590,7 @@ static hipError_t launch_pair_costs(const KParams& P, int view, int slot, int n,
 }
 template <int WR, int WSTEP>
 static const void* half_symbol() {
-    return reinterpret_cast<const void*>(&k_half<WR, WSTEP>);
+    return reinterpret_cast<const void*>(&k_half<WR, WSTEP, false>);
 }
 
 template <int WR, int WSTEP>

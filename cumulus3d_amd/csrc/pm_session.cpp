@@ -447,7 +447,7 @@ int c3d_pm_begin(c3d_pm_backend* be, const c3d_kernel_input* in, c3d_pm_session*
         for (int l = 0; l < nl; ++l) {
             if (!kv.levels[l].gray || kv.levels[l].width < 1 || kv.levels[l].height < 1)
                 return fail(C3D_INVALID_ARGUMENT, "empty level image");
-            // source-sample table indices are formed exactly in FP32
+            // source-sample table byte offsets are formed exactly in FP32 (multiples of 4 below 2^26)
             if (((uint64_t)kv.levels[l].width + 3) * ((uint64_t)kv.levels[l].height + 3) > (1ULL << 24))
                 return fail(C3D_INVALID_ARGUMENT, "image larger than (w + 3)(h + 3) <= 2^24 pixels");
         }

@@ -39,16 +39,16 @@ class PmError(RuntimeError):
         self.message = message
 
 
-def library_path(repro: bool = False) -> str:
-    return os.path.join(_LIB_DIR, "libc3d_pm_repro.so" if repro else "libc3d_pm.so")
+def library_path() -> str:
+    return os.path.join(_LIB_DIR, "libc3d_pm.so")
 
 
-def load(repro: bool = False) -> C.CDLL:
+def load() -> C.CDLL:
     """Loads the engine library (built by cumulus3d_amd/build.py); raises if it is missing."""
     with _lock:
-        if repro in _libs:
-            return _libs[repro]
-        path = library_path(repro)
+        if "engine" in _libs:
+            return _libs["engine"]
+        path = library_path()
         if not os.path.exists(path):
             raise PmError(abi.C3D_NOT_AVAILABLE, f"{path} is missing: build the extension first (__graft_entry__.build())")
         L = C.CDLL(path)
@@ -79,7 +79,7 @@ def load(repro: bool = False) -> C.CDLL:
         for n in ("c3d_pm_create", "c3d_pm_begin", "c3d_pm_run", "c3d_pm_evaluate", "c3d_pm_filter", "c3d_pm_upsample",
                   "c3d_pm_median", "c3d_pm_densify", "c3d_pm_pair_costs", "c3d_pm_is_available"):
             getattr(L, n).restype = C.c_int
-        _libs[repro] = L
+        _libs["engine"] = L
         return L
 
 
@@ -88,16 +88,16 @@ def _check(L, rc: int):
         raise PmError(rc, L.c3d_pm_last_error().decode(errors="replace"))
 
 
-def is_available(repro: bool = False) -> bool:
+def is_available() -> bool:
     """is_available (crates/cuda/src/device.rs:80-89): a gfx950 device can be opened."""
-    return bool(load(repro).c3d_pm_is_available())
+    return bool(load().c3d_pm_is_available())
 
 
 class HipPatchMatch:
     """PatchMatchBackend over device `ordinal` (CudaPatchMatch::new / try_default, patchmatch.rs:239-246)."""
 
-    def __init__(self, ordinal: int = 0, options: Optional[abi.c3d_pm_options] = None, repro: bool = False):
-        self.L = load(repro)
+    def __init__(self, ordinal: int = 0, options: Optional[abi.c3d_pm_options] = None):
+        self.L = load()
         h = C.c_void_p()
         _check(self.L, self.L.c3d_pm_create(ordinal, C.byref(options) if options is not None else None, C.byref(h)))
         self.h = h

@@ -110,7 +110,7 @@ def _check(rc: int):
 class Oracle:
     """CPU restatement of a PatchStereo session over a KernelInput (same operations as the C ABI)."""
 
-    def __init__(self, inp: abi.KernelInput, threads: Optional[int] = None, rcp_mode: int = 0):
+    def __init__(self, inp: abi.KernelInput, threads: Optional[int] = None, rcp_mode: int = 1):
         self.inp = inp
         L = lib()
         L.pmo_set_rcp_mode(rcp_mode)

@@ -20,8 +20,8 @@
  * in gray levels (0..255) and shifted by the reference centre value before the NCC sums, and a
  * two-pass weighted reference variance. These are exact algebraic restatements of the reference's
  * formulas (NCC is invariant to the 1/255 scale and to the shift); they change only rounding.
- * The per-sample reciprocal of the projective divide is either IEEE (pmo mode 0) or the
- * three-step Newton reciprocal (mode 1), matching the device's two build modes.
+ * The per-sample reciprocal of the projective divide is the device's three-step Newton reciprocal
+ * (mode 1, the default); mode 0 (IEEE division) is kept to measure what that choice changes.
  *
  * Pinning (what fixes each part independently of this file):
  *   hash_bytes, mix64, rng_*     published FNV-1a-64 / splitmix64 vectors (tests/test_oracle_kats.py)
@@ -63,7 +63,7 @@ static __thread char g_err[512];
 static void set_err(const char* m) { snprintf(g_err, sizeof g_err, "%s", m); }
 const char* pmo_last_error(void) { return g_err; }
 
-static int g_rcp_mode = 0; /* 0: IEEE 1/z, 1: Newton reciprocal */
+static int g_rcp_mode = 1; /* 0: IEEE 1/z, 1: Newton reciprocal (the device's) */
 void pmo_set_rcp_mode(int mode) { g_rcp_mode = mode; }
 
 /* ================================================================ integer primitives */

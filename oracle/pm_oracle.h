@@ -52,8 +52,8 @@ double pmo_quantile_sorted(const double* v, size_t n, double q);
 float pmo_bilinear_clamped(const float* data, size_t w, size_t h, float x, float y);
 
 /* the device's elementary functions, restated */
-/* per-sample reciprocal of the projective divide: 0 = IEEE division, 1 = three-step Newton
- * reciprocal from a bit-pattern seed (the device's reproducible build) */
+/* per-sample reciprocal of the projective divide: 1 = three-step Newton reciprocal from a
+ * bit-pattern seed (the device's; default), 0 = IEEE division */
 void pmo_set_rcp_mode(int mode);
 float pmo_expf(float x);
 void pmo_sincosf(float x, float* s, float* c);

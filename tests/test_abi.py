@@ -28,9 +28,8 @@ def test_header_declares_the_trait_surface():
         assert n in names
 
 
-@pytest.mark.parametrize("repro", [False, True])
-def test_library_exports_every_declared_symbol(repro):
-    L = C.CDLL(pm.library_path(repro))
+def test_library_exports_every_declared_symbol():
+    L = C.CDLL(pm.library_path())
     missing = [n for n in declared_functions() if not hasattr(L, n)]
     assert not missing, missing
     assert sorted(pm.EXPORTS) == declared_functions()

@@ -6,10 +6,12 @@ each view using the other two as sources, window radius 5 step 2, 5 iterations, 
 
 Bars:
   * per-hypothesis NCC cost within 1e-4 absolute of the FP64 evaluation (north_star tolerance);
-  * the reproducible build (C3D_REPRO_RCP, Newton reciprocal) equals the FP32 oracle bit for bit
-    for costs, runs, evaluation, read-out, upsampling and the median filter;
-  * the production build (v_rcp_f32) agrees with the oracle within 1 % relative depth on >= 99 %
-    of valid pixels, normals within 2 deg on >= 99 % (north_star tolerance).
+  * the engine equals the FP32 oracle bit for bit for costs, runs, evaluation, read-out,
+    upsampling, the median filter and the whole device-resident schedule;
+  * against the oracle computing the per-sample projective divide with IEEE division (the
+    reference's literal formula; the engine uses a reproducible Newton reciprocal) depth agrees
+    within 1 % relative on >= 99 % of valid pixels and normals within 2 deg on >= 99 %
+    (north_star tolerance).
 """
 import numpy as np
 import pytest
@@ -89,9 +91,9 @@ def test_pair_costs_match_fp64(c1):
                 assert ((ref == 2.0) != (got == 2.0)).sum() <= 2
 
 
-def test_pair_costs_repro_bit_exact(c1):
+def test_pair_costs_bit_exact(c1):
     sc, inp = c1
-    be = pm.HipPatchMatch(repro=True)
+    be = pm.HipPatchMatch()
     o = _oracle(inp, 1)
     with be.begin(inp) as s:
         xy, planes = _queries(sc, 0, 2000, seed=3)
@@ -116,10 +118,10 @@ def _agreement(d_gpu, d_ref, n_gpu, n_ref):
     return valid.mean(), (rel <= 0.01).mean(), (ang <= 2.0).mean()
 
 
-def test_run_repro_bit_exact(c1):
+def test_run_bit_exact(c1):
     """Random-init photometric run, 5 iterations (config 1): identical to the oracle."""
     sc, inp = c1
-    be = pm.HipPatchMatch(repro=True)
+    be = pm.HipPatchMatch()
     o = _oracle(inp, 1)
     p = abi.run_params(0, False, True, 5, 1, False)
     g = _states(inp, [0, 1, 2])
@@ -133,7 +135,7 @@ def test_run_repro_bit_exact(c1):
         assert np.array_equal(a.cost.view(np.uint32), b.cost.view(np.uint32))
 
 
-def test_run_production_within_tolerance(c1):
+def test_run_within_tolerance_of_ieee_divide(c1):
     sc, inp = c1
     be = pm.HipPatchMatch()
     o = _oracle(inp, 0)
@@ -149,9 +151,9 @@ def test_run_production_within_tolerance(c1):
         assert dfrac >= 0.99 and nfrac >= 0.99
 
 
-def test_evaluate_filter_repro_bit_exact(small):
+def test_evaluate_filter_bit_exact(small):
     sc, inp = small
-    be = pm.HipPatchMatch(repro=True)
+    be = pm.HipPatchMatch()
     o = _oracle(inp, 1)
     views = [0, 1, 2]
     g = _states(inp, views)
@@ -179,7 +181,7 @@ def test_evaluate_filter_repro_bit_exact(small):
     assert sum(int(c.sum()) for c in cnt) > 0
 
 
-def test_densify_production_within_tolerance(small):
+def test_densify_within_tolerance_of_ieee_divide(small):
     """Device-resident single-level schedule (photometric 5, one geometric round of 2, read-out)."""
     sc, inp = small
     be = pm.HipPatchMatch()
@@ -196,10 +198,10 @@ def test_densify_production_within_tolerance(small):
         assert (gc[v] == rc[v]).mean() > 0.98
 
 
-def test_densify_repro_matches_trait_calls(small):
-    """c3d_pm_densify equals the same schedule driven through run/filter (and the oracle) bit for bit."""
+def test_densify_bit_exact(small):
+    """c3d_pm_densify (states resident on the device) equals the oracle's schedule bit for bit."""
     sc, inp = small
-    be = pm.HipPatchMatch(repro=True)
+    be = pm.HipPatchMatch()
     o = _oracle(inp, 1)
     with be.begin(inp) as s:
         go, gc, graw, tm = s.densify([0, 1, 2], 3, 1, 2)
@@ -229,9 +231,9 @@ def _two_level(seed=5):
     return sc, scenes.kernel_input(sc, SRC3, [(20.0, 80.0)] * 3, pmp, fl, levels=levels)
 
 
-def test_upsample_median_repro_bit_exact():
+def test_upsample_median_bit_exact():
     sc, inp = _two_level()
-    be = pm.HipPatchMatch(repro=True)
+    be = pm.HipPatchMatch()
     o = _oracle(inp, 1)
     views = [0, 1, 2]
     low = _states(inp, views, 0)
