@@ -231,13 +231,13 @@ struct Win {
     static constexpr int LAST = -WR + (WN - 1) * WSTEP;  // last sample offset
 };
 
-// Weighted statistics of the reference patch (patchmatch.cu:199-223): bilateral weights w_k and
-// weighted deviations wd_k = w_k (I_k - mu) in gray levels, two-pass variance.
+// Weighted statistics of the reference patch (patchmatch.cu:199-223): bilateral weights w_k in
+// registers, weighted mean and two-pass variance in gray levels. The window values I_k themselves
+// are re-read from the reference accessor (LDS) where the covariance needs them.
 template <int WR, int WSTEP>
 struct RefWin {
-    float ic, W, iW, var;
+    float ic, mu, W, iW, var;
     float w[Win<WR, WSTEP>::WN2];
-    float wd[Win<WR, WSTEP>::WN2];
 };
 
 // Reference image accessors: 0 outside the image.
@@ -289,10 +289,10 @@ __device__ __forceinline__ void ref_stats(const KParams& P, const REF& ref, int 
             const int k = a * WN + b;
             float dk = ref.at(x + dx, y + dy) - mu;
             float wd = rw.w[k] * dk;
-            rw.wd[k] = wd;
             V = fmaf(wd, dk, V);
         }
     }
+    rw.mu = mu;
     rw.W = W;
     rw.iW = 1.0f / W;
     rw.var = V / W;
@@ -344,9 +344,9 @@ __device__ __forceinline__ float quad_sample(const uint32_t* qbase, float qp4, f
 // four window corners have z > 1e-8: z is monotone in each offset (every step is a correctly
 // rounded affine operation), so then every sample does. With CHECK the sample is computed anyway
 // (any position is clamped into the quad table) and replaced by a select.
-template <int WR, int WSTEP, bool CHECK>
-__device__ __forceinline__ void window_sums(const RefWin<WR, WSTEP>& rw, const uint32_t* qbase, float qp4, float koff4, float w1,
-                                            float h1, const float* H, float& Ss, float& SSs, float& IS) {
+template <int WR, int WSTEP, bool CHECK, class REF>
+__device__ __forceinline__ void window_sums(const RefWin<WR, WSTEP>& rw, const REF& ref, int x, int y, const uint32_t* qbase, float qp4,
+                                            float koff4, float w1, float h1, const float* H, float& Ss, float& SSs, float& IS) {
     constexpr int WN = Win<WR, WSTEP>::WN;
     Ss = 0.f;
     SSs = 0.f;
@@ -369,16 +369,16 @@ __device__ __forceinline__ void window_sums(const RefWin<WR, WSTEP>& rw, const u
             float ws = rw.w[k] * sp;
             Ss += ws;
             SSs = fmaf(ws, sp, SSs);
-            IS = fmaf(rw.wd[k], sp, IS);
+            IS = fmaf(ws, ref.at(x + (-WR + b * WSTEP), y + (-WR + a * WSTEP)), IS);
         }
     }
 }
 
-// Bilateral-weighted NCC cost 1 - rho, 2 on failure (patchmatch.cu:244-290).
-template <int WR, int WSTEP>
-__device__ __forceinline__ float ncc_cost(const KParams& P, const RefWin<WR, WSTEP>& rw, const LevelDesc& S, const float* H, int x, int y) {
-    (void)x;
-    (void)y;
+// Bilateral-weighted NCC cost 1 - rho, 2 on failure (patchmatch.cu:244-290); the covariance is
+// sum(w s' I)/W - mu mean(s') with s' the shifted source samples.
+template <int WR, int WSTEP, class REF>
+__device__ __forceinline__ float ncc_cost(const KParams& P, const RefWin<WR, WSTEP>& rw, const REF& ref, const LevelDesc& S,
+                                          const float* H, int x, int y) {
     float z = H[8];
     if (!(z > 1e-8f)) return 2.0f;
     float iz = 1.0f / z;
@@ -394,13 +394,13 @@ __device__ __forceinline__ float ncc_cost(const KParams& P, const RefWin<WR, WST
     const bool all_front = fmaf(H[6], lo, za) > 1e-8f && fmaf(H[6], hi, za) > 1e-8f && fmaf(H[6], lo, zb) > 1e-8f &&
                            fmaf(H[6], hi, zb) > 1e-8f;
     float Ss, SSs, IS;
-    if (all_front) window_sums<WR, WSTEP, false>(rw, qbase, qp4, koff4, w1, h1, H, Ss, SSs, IS);
-    else window_sums<WR, WSTEP, true>(rw, qbase, qp4, koff4, w1, h1, H, Ss, SSs, IS);
+    if (all_front) window_sums<WR, WSTEP, false>(rw, ref, x, y, qbase, qp4, koff4, w1, h1, H, Ss, SSs, IS);
+    else window_sums<WR, WSTEP, true>(rw, ref, x, y, qbase, qp4, koff4, w1, h1, H, Ss, SSs, IS);
     const float iW = rw.iW;
     float mus = Ss * iW;
     float vars = fmaf(-mus, mus, SSs * iW);
     if (vars < P.vmin255) return 2.0f;
-    float cov = IS * iW;
+    float cov = fmaf(-rw.mu, mus, IS * iW);
     float cc = 1.0f - cov / sqrtf(rw.var * vars);
     return fminf(fmaxf(cc, 0.0f), 2.0f);
 }
@@ -431,12 +431,13 @@ __device__ __forceinline__ float geo_error(const KParams& P, const LevelDesc& R,
 }
 
 // (photometric cost, truncated geometric error) of one hypothesis x one source (patchmatch.cu:316-324)
-template <int WR, int WSTEP>
-__device__ __forceinline__ float pair_cost(const KParams& P, const RefWin<WR, WSTEP>& rw, const LevelDesc& R, const LevelDesc& S,
-                                           const PairRec& pr, const LevelPair& lp, int x, int y, float4 h, float* geo) {
+template <int WR, int WSTEP, class REF>
+__device__ __forceinline__ float pair_cost(const KParams& P, const RefWin<WR, WSTEP>& rw, const REF& ref, const LevelDesc& R,
+                                           const LevelDesc& S, const PairRec& pr, const LevelPair& lp, int x, int y, float4 h,
+                                           float* geo) {
     float H[9];
     float c = 2.0f;
-    if (homography(R, lp, 1.0f / R.fx, 1.0f / R.fy, (float)x, (float)y, h.w, xyz(h), H)) c = ncc_cost<WR, WSTEP>(P, rw, S, H, x, y);
+    if (homography(R, lp, 1.0f / R.fx, 1.0f / R.fy, (float)x, (float)y, h.w, xyz(h), H)) c = ncc_cost<WR, WSTEP>(P, rw, ref, S, H, x, y);
     if (P.geometric) *geo = fminf(geo_error(P, R, S, pr, x, y, h.w), P.gmax);
     return c;
 }

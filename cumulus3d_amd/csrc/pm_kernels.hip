@@ -43,13 +43,13 @@ __device__ __forceinline__ float4 pick4(const float4 (&a)[N], int i) {
 
 // (photometric cost, truncated geometric error) of one hypothesis x one source with the pixel's
 // reciprocal focal lengths hoisted (patchmatch.cu:316-324)
-template <int WR, int WSTEP, bool GEO>
-__device__ __forceinline__ float pair_cost_h(const KParams& P, const RefWin<WR, WSTEP>& rw, const LevelDesc& R, const LevelDesc& S,
-                                             const PairRec& pr, const LevelPair& lp, float ifx, float ify, int x, int y, float4 h,
-                                             float& geo) {
+template <int WR, int WSTEP, bool GEO, class REF>
+__device__ __forceinline__ float pair_cost_h(const KParams& P, const RefWin<WR, WSTEP>& rw, const REF& ref, const LevelDesc& R,
+                                             const LevelDesc& S, const PairRec& pr, const LevelPair& lp, float ifx, float ify, int x,
+                                             int y, float4 h, float& geo) {
     float H[9];
     float c = 2.0f;
-    if (homography(R, lp, ifx, ify, (float)x, (float)y, h.w, xyz(h), H)) c = ncc_cost<WR, WSTEP>(P, rw, S, H, x, y);
+    if (homography(R, lp, ifx, ify, (float)x, (float)y, h.w, xyz(h), H)) c = ncc_cost<WR, WSTEP>(P, rw, ref, S, H, x, y);
     geo = GEO ? fminf(geo_error(P, R, S, pr, x, y, h.w), P.gmax) : 0.f;
     return c;
 }
@@ -167,7 +167,7 @@ __global__ __launch_bounds__(256) __attribute__((amdgpu_waves_per_eu(2, 2))) voi
 #pragma nounroll
         for (int i = 0; i < 9; ++i) {
             float gi;
-            const float mi = pair_cost_h<WR, WSTEP, GEO>(P, rw, R, S, pr, lp, ifx, ify, x, y, hyp[i * 256], gi);
+            const float mi = pair_cost_h<WR, WSTEP, GEO>(P, rw, ref, R, S, pr, lp, ifx, ify, x, y, hyp[i * 256], gi);
 #pragma unroll
             for (int q = 0; q < 9; ++q)
                 if (q == i) {
@@ -267,7 +267,7 @@ __global__ __launch_bounds__(256) __attribute__((amdgpu_waves_per_eu(2, 2))) voi
                 const PairRec& pr = prs[j];
                 const LevelDesc S = P.lv[pr.src * P.nlevels + P.level];
                 float g;
-                float m = pair_cost_h<WR, WSTEP, GEO>(P, rw, R, S, pr, lps[j], ifx, ify, x, y, hc, g);
+                float m = pair_cost_h<WR, WSTEP, GEO>(P, rw, ref, R, S, pr, lps[j], ifx, ify, x, y, hc, g);
                 nm += wj * (m + P.lambda * g);
             }
             cc = nm / den;
@@ -282,15 +282,31 @@ __global__ __launch_bounds__(256) __attribute__((amdgpu_waves_per_eu(2, 2))) voi
     P.bview[base + pix] = (uint8_t)vbest;
 }
 
+// Stages the reference rows of a 64 x 4 pixel tile (plus the window margin) in LDS.
+template <int WR>
+__device__ __forceinline__ void stage_pix_tile(float* tile, const uint8_t* g, int w, int h, int px0, int py0) {
+    constexpr int TW = PIX_TW + 2 * WR, TH = PIX_TH + 2 * WR;
+    const int tx0 = px0 - WR, ty0 = py0 - WR;
+    for (int i = threadIdx.x; i < TW * TH; i += 256) {
+        int ty = i / TW, tx = i - ty * TW;
+        int gx = tx0 + tx, gy = ty0 + ty;
+        tile[i] = (gx >= 0 && gy >= 0 && gx < w && gy < h) ? (float)g[(size_t)gy * w + gx] : 0.0f;
+    }
+}
+
 // ================================================================ run start / evaluation (patchmatch.cu:659-692)
 template <int WR, int WSTEP>
 __global__ __launch_bounds__(256) void k_eval(KParams P, int mode) {
+    constexpr int TW = PIX_TW + 2 * WR;
+    __shared__ float tile[TW * (PIX_TH + 2 * WR)];
     BatchItem it;
     int px0, py0;
     if (!locate_tile(P, xcd_tile(blockIdx.x, P.ntiles), PIX_TW, PIX_TH, it, px0, py0)) return;
+    const LevelDesc R = P.lv[it.view * P.nlevels + P.level];
+    stage_pix_tile<WR>(tile, P.gray + R.goff, it.w, it.h, px0, py0);
+    __syncthreads();
     const int x = px0 + (threadIdx.x & 63), y = py0 + (threadIdx.x >> 6);
     if (x >= it.w || y >= it.h) return;
-    const LevelDesc R = P.lv[it.view * P.nlevels + P.level];
     const ViewDesc vd = P.views[it.view];
     const PairRec* prs = P.pairs + (size_t)it.view * MAXSRC;
     const LevelPair* lps = P.lpairs + ((size_t)it.view * P.nlevels + P.level) * MAXSRC;
@@ -307,9 +323,9 @@ __global__ __launch_bounds__(256) void k_eval(KParams P, int mode) {
     }
     float c = 2.f;
     if (h.w > 0.f) {
-        RefGlobal g{P.gray + R.goff, it.w, it.h};
+        RefTile<TW> ref{tile, px0 - WR, py0 - WR};
         RefWin<WR, WSTEP> rw;
-        ref_stats<WR, WSTEP>(P, g, x, y, rw);
+        ref_stats<WR, WSTEP>(P, ref, x, y, rw);
         // top-K mean (patchmatch.cu:326-345)
         float b0 = 1e30f, b1 = 1e30f, b2 = 1e30f;
 #pragma nounroll
@@ -317,7 +333,7 @@ __global__ __launch_bounds__(256) void k_eval(KParams P, int mode) {
             const PairRec& pr = prs[j];
             const LevelDesc S = P.lv[pr.src * P.nlevels + P.level];
             float gq = 0.f;
-            float cc = pair_cost<WR, WSTEP>(P, rw, R, S, pr, lps[j], x, y, h, &gq);
+            float cc = pair_cost<WR, WSTEP>(P, rw, ref, R, S, pr, lps[j], x, y, h, &gq);
             if (P.geometric) cc = cc + P.lambda * gq;
             if (cc < b0) {
                 b2 = b1;
@@ -343,12 +359,16 @@ __global__ __launch_bounds__(256) void k_eval(KParams P, int mode) {
 // ================================================================ final read-out (patchmatch.cu:694-737)
 template <int WR, int WSTEP>
 __global__ __launch_bounds__(256) void k_filter(KParams P) {
+    constexpr int TW = PIX_TW + 2 * WR;
+    __shared__ float tile[TW * (PIX_TH + 2 * WR)];
     BatchItem it;
     int px0, py0;
     if (!locate_tile(P, xcd_tile(blockIdx.x, P.ntiles), PIX_TW, PIX_TH, it, px0, py0)) return;
+    const LevelDesc R = P.lv[it.view * P.nlevels + P.level];
+    stage_pix_tile<WR>(tile, P.gray + R.goff, it.w, it.h, px0, py0);
+    __syncthreads();
     const int x = px0 + (threadIdx.x & 63), y = py0 + (threadIdx.x >> 6);
     if (x >= it.w || y >= it.h) return;
-    const LevelDesc R = P.lv[it.view * P.nlevels + P.level];
     const ViewDesc vd = P.views[it.view];
     const PairRec* prs = P.pairs + (size_t)it.view * MAXSRC;
     const LevelPair* lps = P.lpairs + ((size_t)it.view * P.nlevels + P.level) * MAXSRC;
@@ -357,9 +377,9 @@ __global__ __launch_bounds__(256) void k_filter(KParams P) {
     float4 h = P.plane[base + pix];
     int count = 0;
     if (h.w > 0.f) {
-        RefGlobal g{P.gray + R.goff, it.w, it.h};
+        RefTile<TW> ref{tile, px0 - WR, py0 - WR};
         RefWin<WR, WSTEP> rw;
-        ref_stats<WR, WSTEP>(P, g, x, y, rw);
+        ref_stats<WR, WSTEP>(P, ref, x, y, rw);
         f3 ray = pixel_ray(R, x, y);
         f3 X = mk3(ray.x * h.w, ray.y * h.w, h.w);
         float lx = sqrtf(dot3(X, X));
@@ -376,7 +396,7 @@ __global__ __launch_bounds__(256) void k_filter(KParams P) {
             if (cphi <= 0.f) continue;
             float H[9];
             if (!homography(R, lps[j], 1.0f / R.fx, 1.0f / R.fy, (float)x, (float)y, h.w, xyz(h), H)) continue;
-            float m = ncc_cost<WR, WSTEP>(P, rw, S, H, x, y);
+            float m = ncc_cost<WR, WSTEP>(P, rw, ref, S, H, x, y);
             float e = P.ncc_norm * c3d_expf(-m * m * P.inv2ncc);
             if (e / (e + 0.5f) < P.q_min) continue;
             if (geo_error(P, R, S, pr, x, y, h.w) > P.filter_gmax) continue;
@@ -403,7 +423,7 @@ __global__ __launch_bounds__(256) void k_pair_costs(KParams P, int view, int slo
     float4 h = planes[i];
     float H[9];
     float c = 2.f;
-    if (homography(R, lp, 1.0f / R.fx, 1.0f / R.fy, (float)x, (float)y, h.w, xyz(h), H)) c = ncc_cost<WR, WSTEP>(P, rw, S, H, x, y);
+    if (homography(R, lp, 1.0f / R.fx, 1.0f / R.fy, (float)x, (float)y, h.w, xyz(h), H)) c = ncc_cost<WR, WSTEP>(P, rw, g, S, H, x, y);
     out[i] = c;
 }
 

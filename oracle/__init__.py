@@ -37,7 +37,14 @@ def lib():
     if _lib is None:
         if not os.path.exists(LIB_PATH):
             build()
-        L = C.CDLL(LIB_PATH)
+        _lib = load_library(LIB_PATH)
+    return _lib
+
+
+def load_library(path: str):
+    """Loads an oracle build at `path` and declares its signatures."""
+    if True:
+        L = C.CDLL(path)
         u32p = C.POINTER(C.c_uint32)
         vsp = C.POINTER(abi.c3d_view_state)
         L.pmo_hash_bytes.argtypes = [C.c_char_p, C.c_size_t]
@@ -94,8 +101,7 @@ def lib():
         L.pmo_ncc_cost_f64.restype = C.c_double
         for n in ("pmo_run", "pmo_evaluate", "pmo_filter", "pmo_upsample", "pmo_median", "pmo_densify", "pmo_run_rows"):
             getattr(L, n).restype = C.c_int
-        _lib = L
-    return _lib
+        return L
 
 
 class OracleError(RuntimeError):

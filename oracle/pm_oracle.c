@@ -17,8 +17,8 @@
  * Arithmetic (FP32 mode) follows the device implementation exactly: explicit fmaf where the device
  * uses FMA, plain IEEE operations elsewhere (compiled with -ffp-contract=off), IEEE division and
  * square root, the same polynomial exp (pmo_expf) and sin/cos (pmo_sincosf), source samples kept
- * in gray levels (0..255) and shifted by the reference centre value before the NCC sums, and a
- * two-pass weighted reference variance. These are exact algebraic restatements of the reference's
+ * in gray levels (0..255) and shifted by the reference centre value before the NCC sums, a
+ * two-pass weighted reference variance, and the covariance as sum(w s' I)/W - mu mean(s'). These are exact algebraic restatements of the reference's
  * formulas (NCC is invariant to the 1/255 scale and to the shift); they change only rounding.
  * The per-sample reciprocal of the projective divide is the device's three-step Newton reciprocal
  * (mode 1, the default); mode 0 (IEEE division) is kept to measure what that choice changes.
@@ -53,6 +53,12 @@
 #endif
 
 #define MAXSRC C3D_MAX_SOURCES
+
+/* PMO_MUT selects a deliberately planted mistake for tests/test_oracle_mutations.py (0 = none).
+ * Only test builds set it; the shipped oracle is compiled without it. */
+#ifndef PMO_MUT
+#define PMO_MUT 0
+#endif
 #define INF_ERR 1e30f
 #define PI_F 3.14159265358979323846f
 
@@ -73,7 +79,7 @@ uint64_t pmo_hash_bytes(const uint8_t* b, size_t n) {
     uint64_t h = 0xcbf29ce484222325ULL;
     for (size_t i = 0; i < n; ++i) {
         h ^= (uint64_t)b[i];
-        h *= 0x00000100000001b3ULL;
+        h *= PMO_MUT == 10 ? 0x00000100000001b5ULL : 0x00000100000001b3ULL;
     }
     return h;
 }
@@ -88,7 +94,7 @@ uint64_t pmo_mix64(uint64_t z) {
 /* per-pixel start state (kernel.rs:248-252, patchmatch.cu:122-125) */
 uint64_t pmo_rng_state(uint64_t seed, uint64_t key, uint32_t level, uint32_t run, uint32_t t,
                        uint32_t half, uint64_t pixel) {
-    uint64_t tag = ((uint64_t)level << 48) ^ ((uint64_t)run << 32) ^ ((uint64_t)t << 16) ^ (uint64_t)half;
+    uint64_t tag = ((uint64_t)level << 48) ^ ((uint64_t)run << 32) ^ ((uint64_t)t << (PMO_MUT == 1 ? 15 : 16)) ^ (uint64_t)half;
     return pmo_mix64(seed ^ pmo_mix64(key ^ pmo_mix64(tag)) ^ pixel);
 }
 
@@ -121,7 +127,7 @@ double pmo_emission_norm(double sigma) {
 }
 /* E(c) (math.rs:73-76) */
 double pmo_emission(double cost, double sigma) {
-    return pmo_emission_norm(sigma) * exp(-cost * cost / (2.0 * sigma * sigma));
+    return pmo_emission_norm(sigma) * exp(-cost * cost / (2.0 * sigma * (PMO_MUT == 2 ? 1.0 : sigma)));
 }
 /* E/(E+0.5) (math.rs:78-82) */
 double pmo_visibility_probability(double cost, double sigma) {
@@ -133,7 +139,7 @@ double pmo_triangulation_prior(double cos_theta, double min_angle_rad) {
     double cmin = cos(min_angle_rad);
     if (cos_theta > cmin) {
         double s = 1.0 - (1.0 - cos_theta) / (1.0 - cmin);
-        double v = 1.0 - s * s;
+        double v = PMO_MUT == 9 ? 1.0 + s * s : 1.0 - s * s;
         return v < 0 ? 0 : (v > 1 ? 1 : v);
     }
     return 1.0;
@@ -207,7 +213,7 @@ float pmo_expf(float x) {
     float n = rintf(x * 1.44269504088896341f);
     float r = fmaf(-n, 0.693145751953125f, x);
     r = fmaf(-n, 1.42860677e-06f, r);
-    float p = 1.98412698e-04f;           /* 1/5040 */
+    float p = PMO_MUT == 7 ? 0.0f : 1.98412698e-04f; /* 1/5040 */
     p = fmaf(p, r, 1.38888889e-03f);     /* 1/720 */
     p = fmaf(p, r, 8.33333333e-03f);     /* 1/120 */
     p = fmaf(p, r, 4.16666667e-02f);     /* 1/24 */
@@ -271,7 +277,7 @@ static void level_pair_d(const float kr[4], const float kj[4], const float r[9],
     for (int i = 0; i < 3; ++i)
         for (int k = 0; k < 3; ++k) {
             double s = 0.0;
-            for (int m = 0; m < 3; ++m) s += kjm[i][m] * (double)r[m * 3 + k];
+            for (int m = 0; m < 3; ++m) s += kjm[i][m] * (double)(PMO_MUT == 3 ? r[k * 3 + m] : r[m * 3 + k]);
             kr_[i][k] = s;
         }
     for (int i = 0; i < 3; ++i)
@@ -534,16 +540,17 @@ static inline float ref_at(const lvl_t* R, int x, int y) {
 
 #define MAXWN2 (41 * 41)
 
-/* reference patch statistics (patchmatch.cu:199-223), gray-level units, two-pass variance */
+/* reference patch statistics (patchmatch.cu:199-223), gray-level units, two-pass variance; the
+ * window values ik are kept for the covariance sum */
 typedef struct {
     float ic, W, mu, var;
     float w[MAXWN2];
-    float wd[MAXWN2];
+    float ik[MAXWN2];
 } rstats_t;
 
 static void ref_stats(const pmo_ctx* c, const lvl_t* R, int x, int y, rstats_t* s) {
     const int WN = c->wn;
-    float ik[MAXWN2];
+    float* ik = s->ik;
     s->ic = ref_at(R, x, y);
     float W = 0.0f, S = 0.0f;
     for (int a = 0; a < WN; ++a) {
@@ -566,7 +573,6 @@ static void ref_stats(const pmo_ctx* c, const lvl_t* R, int x, int y, rstats_t* 
     for (int k = 0; k < WN * WN; ++k) {
         float dk = ik[k] - mu;
         float wd = s->w[k] * dk;
-        s->wd[k] = wd;
         V = fmaf(wd, dk, V);
     }
     s->W = W;
@@ -585,6 +591,11 @@ static int homography(const lvl_t* R, const lpair_t* L, float x, float y, float 
     if (!(delta < -1e-12f)) return 0;
     float id = 1.0f / delta, invd = 1.0f / d;
     float u0 = n.x * ifx, u1 = n.y * ify;
+    if (PMO_MUT == 8) {
+        float t_ = u0;
+        u0 = u1;
+        u1 = t_;
+    }
     for (int r = 0; r < 3; ++r) {
         float br = L->b[r] * id;
         H[r * 3 + 0] = L->A[r * 3 + 0] + br * u0;
@@ -650,14 +661,14 @@ static float ncc_cost(const pmo_ctx* c, const rstats_t* rs, const lvl_t* S, cons
             float ws = rs->w[k] * sp;
             Ss += ws;
             SSs = fmaf(ws, sp, SSs);
-            IS = fmaf(rs->wd[k], sp, IS);
+            IS = fmaf(ws, rs->ik[k], IS);
         }
     }
     float iW = 1.0f / rs->W;
     float mus = Ss * iW;
-    float vars = fmaf(-mus, mus, SSs * iW);
+    float vars = PMO_MUT == 5 ? SSs * iW : fmaf(-mus, mus, SSs * iW);
     if (vars < c->vmin255) return 2.0f;
-    float cov = IS * iW;
+    float cov = fmaf(-rs->mu, mus, IS * iW);
     float cc = 1.0f - cov / sqrtf(rs->var * vars);
     return fminf(fmaxf(cc, 0.0f), 2.0f);
 }
@@ -1563,6 +1574,11 @@ static double src_sample_d(const lvl_t* S, double u, double v) {
             if (xin1) p11 = S->img[(size_t)(y0 + 1) * S->w + x0 + 1];
         }
     }
+    if (PMO_MUT == 4) {
+        double t_ = ax;
+        ax = ay;
+        ay = t_;
+    }
     double top = p00 + ax * (p10 - p00), bot = p01 + ax * (p11 - p01);
     return (top + ay * (bot - top)) / 255.0;
 }
@@ -1628,7 +1644,7 @@ double pmo_ncc_cost_f64(pmo_ctx* c, uint32_t level, uint32_t view, uint32_t slot
     }
     double mus = Ss / W, vars = SSs / W - mus * mus;
     if (vars < 1e-5) return 2.0;
-    double cov = IS / W - mu * mus;
+    double cov = IS / W - (PMO_MUT == 6 ? 0.0 : mu * mus);
     double cc = 1.0 - cov / sqrt(var * vars);
     return cc < 0 ? 0 : (cc > 2 ? 2 : cc);
 }
