@@ -199,16 +199,17 @@ def test_level_pair_homography_projects_plane_points(oracle_lib):
         assert abs(p[0] / p[2] - u) < 1e-3 and abs(p[1] / p[2] - vv) < 1e-3
 
 
-def _identity_input(tex):
+def _identity_input(tex, src_tex=None):
     """Two identical cameras (source = reference): the fronto-parallel plane at any depth maps every
     pixel onto itself, so NCC of identical textures is exactly 1 and the cost 0."""
     h, w = tex.shape
+    src_tex = tex if src_tex is None else src_tex
     f = 200.0
     k = [f, f, (w - 1) / 2.0, (h - 1) / 2.0]
     v = abi.ViewData(key=1, levels=[abi.LevelData(tex, k)], sources=[1],
                      pairs=[(np.eye(3, dtype=np.float32).reshape(9), np.zeros(3, np.float32), np.zeros(3, np.float32))],
                      depth_min=1.0, depth_max=10.0)
-    v2 = abi.ViewData(key=2, levels=[abi.LevelData(tex, k)], sources=[0],
+    v2 = abi.ViewData(key=2, levels=[abi.LevelData(src_tex, k)], sources=[0],
                       pairs=[(np.eye(3, dtype=np.float32).reshape(9), np.zeros(3, np.float32), np.zeros(3, np.float32))],
                       depth_min=1.0, depth_max=10.0)
     pm, fl = abi.fast_profile()
@@ -228,6 +229,11 @@ def test_ncc_closed_forms(oracle_lib):
         assert abs(o.ncc_cost_f32(0, 0, 0, x, y, plane)) < 2e-6
     # a normal facing away from the camera: no valid homography -> 2
     assert o.ncc_cost_f32(0, 0, 0, 20, 20, [0.0, 0.0, 1.0, 5.0]) == 2.0
+    # inverted texture: rho = -1 exactly, cost 1 - rho = 2 (a dropped mean term would clamp to 0)
+    oi = oracle.Oracle(_identity_input(tex, (255 - tex).astype(np.uint8)), threads=1)
+    for (x, y) in [(20, 20), (10, 8)]:
+        assert abs(oi.ncc_cost_f64(0, 0, 0, x, y, plane) - 2.0) < 1e-9
+        assert abs(oi.ncc_cost_f32(0, 0, 0, x, y, plane) - 2.0) < 5e-5
     # constant patch -> 2 (variance below 1e-5)
     flat = np.full((40, 48), 90, np.uint8)
     o2 = oracle.Oracle(_identity_input(flat), threads=1)
