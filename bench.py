@@ -120,18 +120,20 @@ def main():
     log(f"scene {args.views} x {args.width}x{args.height} rendered in {time.perf_counter() - t0:.1f} s")
     views = list(range(args.views))
     npix = args.width * args.height
+    sched = abi.Schedule(photometric_iters=args.photometric_iters, geometric_rounds=args.geometric_rounds,
+                         geometric_iters=args.geometric_iters)
     be = pm.HipPatchMatch(0)
     sess = be.begin(inp)
     log(f"session open on {be.name()}")
     for i in range(args.warmup):
-        sess.densify(views, args.photometric_iters, args.geometric_rounds, args.geometric_iters, download=False)
+        sess.densify(views, sched, download=False)
         log(f"warmup {i + 1}/{args.warmup} done")
     device_sync()
     times, tms = [], []
     t_all = time.perf_counter()
     for i in range(args.steps):
         ts = time.perf_counter()
-        _, _, _, tm = sess.densify(views, args.photometric_iters, args.geometric_rounds, args.geometric_iters, download=False)
+        _, _, _, tm = sess.densify(views, sched, download=False)
         times.append(time.perf_counter() - ts)
         tms.append(tm)
         log(f"step {i + 1}/{args.steps}: {times[-1] * 1e3:.1f} ms (device {tm.total_ms:.1f} ms)")

@@ -145,9 +145,15 @@ class c3d_run_params(C.Structure):
 class c3d_schedule(C.Structure):
     _fields_ = [
         ("photometric_iters", C.c_uint32),
+        ("photometric_iters_upper", C.c_uint32),
         ("geometric_rounds", C.c_uint32),
         ("geometric_iters", C.c_uint32),
         ("first_run_id", C.c_uint32),
+        ("restorer_random_init", C.c_int32),
+        ("jbu_sigma_spatial", C.c_float),
+        ("jbu_sigma_color", C.c_float),
+        ("restorer_threshold", C.c_float),
+        ("pad0", C.c_uint32),
     ]
 
 
@@ -164,6 +170,37 @@ class c3d_timing(C.Structure):
 
 
 # ---------------------------------------------------------------- Python-side parameter records
+
+
+@dataclass
+class Schedule:
+    """Per-scale iteration schedule (LevelSchedule, crates/dense/src/params.rs:476-506) and the
+    densify options it reads (jbu sigmas, restorer threshold: params.rs:423-426). The defaults are
+    the fast profile; `quality()` is MvsProfile::Quality."""
+
+    photometric_iters: int = 6
+    photometric_iters_upper: int = 3
+    geometric_rounds: int = 2
+    geometric_iters: int = 2
+    restorer_random_init: bool = False
+    jbu_sigma_spatial: float = 1.0
+    jbu_sigma_color: float = 0.04
+    restorer_threshold: float = 0.1
+    first_run_id: int = 1
+
+    @classmethod
+    def fast(cls) -> "Schedule":
+        return cls()
+
+    @classmethod
+    def quality(cls) -> "Schedule":
+        return cls(photometric_iters=7, photometric_iters_upper=6, geometric_rounds=2, geometric_iters=6,
+                   restorer_random_init=True)
+
+    def c(self) -> c3d_schedule:
+        return c3d_schedule(self.photometric_iters, self.photometric_iters_upper, self.geometric_rounds, self.geometric_iters,
+                            self.first_run_id, int(self.restorer_random_init), self.jbu_sigma_spatial, self.jbu_sigma_color,
+                            self.restorer_threshold, 0)
 
 
 @dataclass

@@ -572,6 +572,52 @@ __global__ void k_threshold(KParams P, const uint8_t* need_per_item, float4* out
     out[p] = h;
 }
 
+// detail restorer merge (densify.rs:396-409): the photometric plane replaces the upsampled one where
+// the upsampled depth is not positive or c_init - c_photo > xi; the merged state carries cost 2 (the
+// upsampled ViewState is ViewState::new, kernel.rs:154-159).
+__global__ void k_restore(float4* plane, const float4* up, const float* c_init, float* cost, float xi, long long n) {
+    long long i = (long long)blockIdx.x * 256 + threadIdx.x;
+    if (i >= n) return;
+    const float4 u = up[i];
+    if (!(u.w <= 0.f || c_init[i] - cost[i] > xi)) plane[i] = u;
+    cost[i] = 2.0f;
+}
+
+// one step of the gray pyramid (build_pyramid and blur3, crates/dense/src/upsample.rs:45-99): the
+// 3-tap Gaussian (edge clamp, rows first, then columns) at the four pixels of the 2x2 cell, their
+// mean, rounded half away from zero and clamped to 8 bits. Every sum runs in the reference's order.
+__global__ void k_pyramid_step(const uint8_t* src, int w, int h, uint8_t* dst, float g0, float g1, float g2) {
+    const int nw = w / 2, nh = h / 2;
+    const int i = blockIdx.x * 256 + threadIdx.x;
+    if (i >= nw * nh) return;
+    const int x = i % nw, y = i / nw;
+    const float g[3] = {g0, g1, g2};
+    float b[4];
+#pragma unroll
+    for (int dy = 0; dy < 2; ++dy)
+#pragma unroll
+        for (int dx = 0; dx < 2; ++dx) {
+            const int X = 2 * x + dx, Y = 2 * y + dy;
+            const int xa = X - 1 < 0 ? 0 : X - 1, xb = X + 1 > w - 1 ? w - 1 : X + 1;
+            float a = 0.0f;
+#pragma unroll
+            for (int k = 0; k < 3; ++k) {
+                int yy = Y + k - 1;
+                yy = yy < 0 ? 0 : (yy > h - 1 ? h - 1 : yy);
+                const uint8_t* r = src + (size_t)yy * w;
+                float t = 0.0f;
+                t += g0 * (float)r[xa];
+                t += g1 * (float)r[X];
+                t += g2 * (float)r[xb];
+                a += g[k] * t;
+            }
+            b[dy * 2 + dx] = a;
+        }
+    const float v = 0.25f * (b[0] + b[1] + b[2] + b[3]);
+    const float r = roundf(v);
+    dst[i] = (uint8_t)(r < 0.f ? 0.f : (r > 255.f ? 255.f : r));
+}
+
 // ================================================================ launchers
 
 int half_tiles(const BatchItem* items, int n) {
@@ -646,6 +692,17 @@ hipError_t launch_quad_build(const uint8_t* g, int w, int h, uint32_t* q, hipStr
 }
 hipError_t launch_extract_depth(const float4* plane, float* out, long long n, hipStream_t s) {
     hipLaunchKernelGGL(k_extract_depth, dim3((unsigned)((n + 255) / 256)), dim3(256), 0, s, plane, out, n);
+    return hipGetLastError();
+}
+hipError_t launch_restore(float4* plane, const float4* up, const float* c_init, float* cost, float xi, long long n,
+                          hipStream_t s) {
+    hipLaunchKernelGGL(k_restore, dim3((unsigned)((n + 255) / 256)), dim3(256), 0, s, plane, up, c_init, cost, xi, n);
+    return hipGetLastError();
+}
+hipError_t launch_pyramid_step(const uint8_t* src, int w, int h, uint8_t* dst, const float g[3], hipStream_t s) {
+    const int n = (w / 2) * (h / 2);
+    if (n == 0) return hipSuccess;
+    hipLaunchKernelGGL(k_pyramid_step, dim3((n + 255) / 256), dim3(256), 0, s, src, w, h, dst, g[0], g[1], g[2]);
     return hipGetLastError();
 }
 hipError_t launch_threshold(const KParams& P, const uint8_t* need, float4* out, hipStream_t s) {

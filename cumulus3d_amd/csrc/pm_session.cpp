@@ -11,7 +11,7 @@
 //   errors             crates/cuda/src/patchmatch.rs:292-294, 366-371, 638-642, 753-756
 // Differences by design: every image is uploaded once as u8 plus a packed quad table (no texture
 // objects); all views of a batch run in one flattened grid; c3d_pm_densify keeps states on the
-// device for the whole single-level schedule.
+// device for the whole multi-scale schedule.
 #include <hip/hip_runtime.h>
 
 #include <algorithm>
@@ -144,6 +144,9 @@ struct c3d_pm_session {
     // states
     DevBuf<float4> plane, prior, plane2;
     DevBuf<float> cost;
+    // device-resident schedule: lower-scale planes, upsampled planes, restorer costs
+    DevBuf<float4> low, up;
+    DevBuf<float> cinit;
     DevBuf<uint8_t> bview, count, need;
     // probes
     DevBuf<int> q_xy;
@@ -158,6 +161,10 @@ struct c3d_pm_session {
     // per-launch timing of the half-step kernels: pairs of events, read after the final sync
     std::vector<hipEvent_t> pool;
     size_t pool_used = 0;
+    // stage boundaries of c3d_pm_densify: an event opens a segment of a stage kind
+    std::vector<hipEvent_t> marks;
+    std::vector<int> mark_kind;
+    size_t marks_used = 0;
 };
 
 namespace {
@@ -371,6 +378,26 @@ int pool_sum(c3d_pm_session* s, double* ms_out, uint32_t* n_out) {
     return C3D_OK;
 }
 
+// Opens a timed segment of stage `kind` (0 photometric, 1 geometric, 2 read-out; -1 closes).
+int mark(c3d_pm_session* s, int kind) {
+    if (s->marks.size() <= s->marks_used) {
+        hipEvent_t e;
+        HIP_TRY(hipEventCreate(&e));
+        s->marks.push_back(e);
+        s->mark_kind.push_back(0);
+    }
+    HIP_TRY(hipEventRecord(s->marks[s->marks_used], s->be->stream));
+    s->mark_kind[s->marks_used] = kind;
+    s->marks_used++;
+    return C3D_OK;
+}
+
+#define MARK(kind)              \
+    do {                        \
+        int m_ = mark(s, kind); \
+        if (m_) return m_;      \
+    } while (0)
+
 }  // namespace
 
 // ================================================================ C ABI
@@ -581,6 +608,9 @@ void c3d_pm_end(c3d_pm_session* s) {
     s->prior.release();
     s->plane2.release();
     s->cost.release();
+    s->low.release();
+    s->up.release();
+    s->cinit.release();
     s->bview.release();
     s->count.release();
     s->need.release();
@@ -590,6 +620,7 @@ void c3d_pm_end(c3d_pm_session* s) {
     for (auto& e : s->ev)
         if (e) (void)hipEventDestroy(e);
     for (auto& e : s->pool) (void)hipEventDestroy(e);
+    for (auto& e : s->marks) (void)hipEventDestroy(e);
     delete s;
 }
 
@@ -789,27 +820,38 @@ int c3d_pm_median(c3d_pm_session* s, uint32_t level, const uint32_t* views, uint
 int c3d_pm_densify(c3d_pm_session* s, const uint32_t* views, uint32_t n, const c3d_schedule* sch, c3d_view_state* out,
                    uint8_t* const* counts, float* const* raw_depth, c3d_timing* timing) {
     if (!s || !sch || !views || n == 0) return fail(C3D_INVALID_ARGUMENT, "null argument");
-    if (s->nl != 1) return fail(C3D_INVALID_ARGUMENT, "c3d_pm_densify handles single-level inputs");
     int rc = set_device(s->be);
     if (rc) return rc;
-    rc = check_views(s, 0, views, n, out);
+    const uint32_t top = (uint32_t)s->nl - 1;
+    rc = check_views(s, top, views, n, out);
     if (rc) return rc;
     for (uint32_t i = 0; i < n; ++i)
         for (uint32_t j = 0; j < i; ++j)
             if (views[i] == views[j]) return fail(C3D_INVALID_ARGUMENT, "duplicate view in c3d_pm_densify");
+    if (s->nl > 1 && !(sch->jbu_sigma_spatial > 0.f && sch->jbu_sigma_color > 0.f))
+        return fail(C3D_INVALID_ARGUMENT, "joint bilateral upsampling needs positive sigmas");
     hipStream_t st = s->be->stream;
-    std::vector<BatchItem> items;
-    long long npix;
-    int max_nsrc;
-    rc = set_batch(s, views, 0, n, 0, items, npix, max_nsrc);
-    if (rc) return rc;
-    HIP_TRY(s->snap.reserve(npix));
-    HIP_TRY(s->plane2.reserve(npix));
+    // every buffer at the largest scale, allocated up front (no reallocation between scales)
+    long long npix_max = 0;
+    for (uint32_t l = 0; l <= top; ++l) {
+        long long p = 0;
+        for (uint32_t i = 0; i < n; ++i) p += (long long)s->lv[views[i] * s->nl + l].w * s->lv[views[i] * s->nl + l].h;
+        npix_max = std::max(npix_max, p);
+    }
+    HIP_TRY(s->plane.reserve(npix_max));
+    HIP_TRY(s->cost.reserve(npix_max));
+    HIP_TRY(s->bview.reserve(npix_max));
+    HIP_TRY(s->count.reserve(npix_max));
+    HIP_TRY(s->snap.reserve(npix_max));
+    HIP_TRY(s->plane2.reserve(npix_max));
     HIP_TRY(s->need.reserve(n));
-    // snapshot offsets: every view of the batch at its batch offset, others absent
-    std::vector<long long> offs((size_t)s->nv, -1);
-    for (auto& it : items) offs[it.view] = it.off;
-    HIP_TRY(hipMemcpyAsync(s->snap_off.p, offs.data(), offs.size() * sizeof(long long), hipMemcpyHostToDevice, st));
+    if (s->nl > 1) {
+        HIP_TRY(s->low.reserve(npix_max));
+        HIP_TRY(s->up.reserve(npix_max));
+        HIP_TRY(s->cinit.reserve(npix_max));
+    }
+    const bool any_prior = s->nl > 1 && s->pm_copy.weak_texture_var > 0.f;
+    if (any_prior) HIP_TRY(s->prior.reserve(npix_max));
     s->snap_valid = false;  // the snapshot buffer now holds device-made maps
     std::vector<uint8_t> need(n);
     for (uint32_t i = 0; i < n; ++i) {
@@ -819,47 +861,107 @@ int c3d_pm_densify(c3d_pm_session* s, const uint32_t* views, uint32_t n, const c
     HIP_TRY(hipMemcpyAsync(s->need.p, need.data(), n, hipMemcpyHostToDevice, st));
     const bool time_halves = timing != nullptr;
     s->pool_used = 0;
-    HIP_TRY(hipEventRecord(s->ev[0], st));
+    s->marks_used = 0;
+    std::vector<std::vector<long long>> snap_offs(top + 1, std::vector<long long>((size_t)s->nv, -1));
+    std::vector<BatchItem> items, prev;
+    long long npix = 0, prev_npix = 0;
+    int max_nsrc = 1;
     uint32_t run_id = sch->first_run_id;
-    KParams k = params_for(s, 0, false, items, max_nsrc);
-    k.run_id = (int)run_id++;
-    k.random_init = 1;
-    rc = launch_run(s, k, items, s->pm_copy, false, sch->photometric_iters, time_halves);
-    if (rc) return rc;
-    HIP_TRY(hipEventRecord(s->ev[1], st));
-    for (uint32_t r = 0; r < sch->geometric_rounds; ++r) {
-        HIP_TRY(launch_extract_depth(s->plane.p, s->snap.p, npix, st));
-        KParams g = params_for(s, 0, true, items, max_nsrc);
-        g.run_id = (int)run_id++;
-        g.random_init = 0;
-        rc = launch_run(s, g, items, s->pm_copy, true, sch->geometric_iters, time_halves);
+    for (uint32_t level = 0; level <= top; ++level) {
+        // params.rs:476-506 through the schedule struct; use_prior densify.rs:366
+        const bool use_prior = level > 0 && any_prior;
+        MARK(0);
+        if (level > 0) {
+            // the lower scale's layout and planes move aside before the new batch layout is written
+            prev = items;
+            prev_npix = npix;
+            HIP_TRY(hipMemcpyAsync(s->low.p, s->plane.p, (size_t)prev_npix * sizeof(float4), hipMemcpyDeviceToDevice, st));
+        }
+        rc = set_batch(s, views, 0, n, level, items, npix, max_nsrc);
         if (rc) return rc;
+        if (level == 0) {
+            KParams k = params_for(s, 0, false, items, max_nsrc);
+            k.run_id = (int)run_id++;
+            k.random_init = 1;
+            rc = launch_run(s, k, items, s->pm_copy, false, sch->photometric_iters, time_halves);
+            if (rc) return rc;
+        } else {
+            // joint bilateral upsampling of the lower scale (patchmatch.cu:741-800)
+            for (size_t i = 0; i < items.size(); ++i) items[i].low_off = (int)prev[i].off;
+            HIP_TRY(hipMemcpyAsync(s->d_batch.p, items.data(), items.size() * sizeof(BatchItem), hipMemcpyHostToDevice, st));
+            KParams u = params_for(s, level, false, items, max_nsrc);
+            u.prior = s->low.p;
+            u.up_inv2s = 1.0f / (2.0f * sch->jbu_sigma_spatial * sch->jbu_sigma_spatial);
+            u.up_inv2c = 1.0f / (2.0f * sch->jbu_sigma_color * sch->jbu_sigma_color);
+            u.ntiles = pix_tiles(items.data(), (int)items.size());
+            HIP_TRY(launch_upsample(u, st));
+            if (use_prior)
+                HIP_TRY(hipMemcpyAsync(s->prior.p, s->plane.p, (size_t)npix * sizeof(float4), hipMemcpyDeviceToDevice, st));
+            HIP_TRY(hipMemcpyAsync(s->up.p, s->plane.p, (size_t)npix * sizeof(float4), hipMemcpyDeviceToDevice, st));
+            // detail restorer (densify.rs:373-409): c_init, photometric run, c_photo, merge
+            KParams e = params_for(s, level, false, items, max_nsrc);
+            e.ntiles = pix_tiles(items.data(), (int)items.size());
+            HIP_TRY(s->ks->eval(e, 0, st));
+            HIP_TRY(hipMemcpyAsync(s->cinit.p, s->cost.p, (size_t)npix * sizeof(float), hipMemcpyDeviceToDevice, st));
+            KParams k = params_for(s, level, false, items, max_nsrc);
+            k.run_id = (int)run_id++;
+            k.random_init = sch->restorer_random_init ? 1 : 0;
+            k.use_prior = use_prior ? 1 : 0;
+            rc = launch_run(s, k, items, s->pm_copy, false, sch->photometric_iters_upper, time_halves);
+            if (rc) return rc;
+            HIP_TRY(s->ks->eval(e, 0, st));
+            HIP_TRY(launch_restore(s->plane.p, s->up.p, s->cinit.p, s->cost.p, sch->restorer_threshold, npix, st));
+        }
+        MARK(1);
+        if (sch->geometric_rounds > 0) {
+            std::vector<long long>& offs = snap_offs[level];
+            for (auto& it : items) offs[it.view] = it.off;
+            HIP_TRY(hipMemcpyAsync(s->snap_off.p, offs.data(), offs.size() * sizeof(long long), hipMemcpyHostToDevice, st));
+        }
+        for (uint32_t r = 0; r < sch->geometric_rounds; ++r) {
+            HIP_TRY(launch_extract_depth(s->plane.p, s->snap.p, npix, st));
+            KParams g = params_for(s, level, true, items, max_nsrc);
+            g.run_id = (int)run_id++;
+            g.random_init = 0;
+            g.use_prior = use_prior ? 1 : 0;
+            rc = launch_run(s, g, items, s->pm_copy, true, sch->geometric_iters, time_halves);
+            if (rc) return rc;
+        }
     }
-    HIP_TRY(hipEventRecord(s->ev[2], st));
+    // read-out at the top scale (densify.rs:428-454)
+    MARK(2);
     if (s->filter_copy.median_filter) {
-        HIP_TRY(s->prior.reserve(npix));
+        HIP_TRY(s->prior.reserve(npix_max));
         HIP_TRY(hipMemcpyAsync(s->prior.p, s->plane.p, (size_t)npix * sizeof(float4), hipMemcpyDeviceToDevice, st));
-        KParams m = params_for(s, 0, false, items, max_nsrc);
+        KParams m = params_for(s, top, false, items, max_nsrc);
         m.ntiles = pix_tiles(items.data(), (int)items.size());
         HIP_TRY(launch_median(m, st));
     }
+    {
+        std::vector<long long>& offs = snap_offs[top];
+        for (auto& it : items) offs[it.view] = it.off;
+        HIP_TRY(hipMemcpyAsync(s->snap_off.p, offs.data(), offs.size() * sizeof(long long), hipMemcpyHostToDevice, st));
+    }
     HIP_TRY(launch_extract_depth(s->plane.p, s->snap.p, npix, st));
-    KParams f = params_for(s, 0, true, items, max_nsrc);
+    KParams f = params_for(s, top, true, items, max_nsrc);
     f.ntiles = pix_tiles(items.data(), (int)items.size());
     HIP_TRY(s->ks->filter(f, st));
     HIP_TRY(launch_threshold(f, s->need.p, s->plane2.p, st));
-    HIP_TRY(hipEventRecord(s->ev[3], st));
-    HIP_TRY(hipEventSynchronize(s->ev[3]));
+    MARK(-1);
+    HIP_TRY(hipEventSynchronize(s->marks[s->marks_used - 1]));
     if (timing) {
-        float a = 0, b = 0, c = 0, d = 0;
-        HIP_TRY(hipEventElapsedTime(&a, s->ev[0], s->ev[1]));
-        HIP_TRY(hipEventElapsedTime(&b, s->ev[1], s->ev[2]));
-        HIP_TRY(hipEventElapsedTime(&c, s->ev[2], s->ev[3]));
-        HIP_TRY(hipEventElapsedTime(&d, s->ev[0], s->ev[3]));
-        timing->photometric_ms = a;
-        timing->geometric_ms = b;
-        timing->filter_ms = c;
-        timing->total_ms = d;
+        double acc[3] = {0.0, 0.0, 0.0};
+        for (size_t i = 0; i + 1 < s->marks_used; ++i) {
+            float ms = 0.f;
+            HIP_TRY(hipEventElapsedTime(&ms, s->marks[i], s->marks[i + 1]));
+            if (s->mark_kind[i] >= 0) acc[s->mark_kind[i]] += ms;
+        }
+        float tot = 0.f;
+        HIP_TRY(hipEventElapsedTime(&tot, s->marks[0], s->marks[s->marks_used - 1]));
+        timing->photometric_ms = acc[0];
+        timing->geometric_ms = acc[1];
+        timing->filter_ms = acc[2];
+        timing->total_ms = tot;
         rc = pool_sum(s, &timing->half_kernel_ms, &timing->half_launches);
         if (rc) return rc;
     }
@@ -875,7 +977,6 @@ int c3d_pm_densify(c3d_pm_session* s, const uint32_t* views, uint32_t n, const c
     }
     if (raw_depth) {
         std::vector<float> hd((size_t)npix);
-        HIP_TRY(launch_extract_depth(s->plane.p, s->snap.p, npix, st));
         HIP_TRY(hipMemcpyAsync(hd.data(), s->snap.p, (size_t)npix * sizeof(float), hipMemcpyDeviceToHost, st));
         HIP_TRY(hipStreamSynchronize(st));
         for (size_t i = 0; i < items.size(); ++i)
@@ -887,6 +988,64 @@ int c3d_pm_densify(c3d_pm_session* s, const uint32_t* views, uint32_t n, const c
         HIP_TRY(hipStreamSynchronize(st));
         for (size_t i = 0; i < items.size(); ++i)
             if (counts[i]) std::memcpy(counts[i], hc.data() + items[i].off, (size_t)items[i].w * items[i].h);
+    }
+    return C3D_OK;
+}
+
+uint32_t c3d_pm_num_levels(uint32_t min_side, uint32_t max_levels, uint32_t min_size) {
+    uint32_t l = 1, sz = min_side;
+    while (l < max_levels && sz / 2 >= min_size) {
+        sz /= 2;
+        l += 1;
+    }
+    return l;
+}
+
+int c3d_pm_build_pyramid(c3d_pm_backend* be, const uint8_t* gray, uint32_t width, uint32_t height, const double k[4],
+                         uint32_t levels, uint8_t* const* out, float* k_out) {
+    if (!be || !gray || !k || !out || !k_out) return fail(C3D_INVALID_ARGUMENT, "null argument");
+    if (levels < 1 || levels > 16) return fail(C3D_INVALID_ARGUMENT, "levels must be 1..16");
+    if (width < 1 || height < 1) return fail(C3D_INVALID_ARGUMENT, "empty image");
+    if ((width >> (levels - 1)) < 1 || (height >> (levels - 1)) < 1)
+        return fail(C3D_INVALID_ARGUMENT, "too many levels for the image size");
+    for (uint32_t l = 0; l < levels; ++l)
+        if (!out[l]) return fail(C3D_INVALID_ARGUMENT, "null level buffer");
+    int rc = set_device(be);
+    if (rc) return rc;
+    // sizes and byte offsets, finest first (reference order is coarsest first)
+    std::vector<uint32_t> ws(levels), hs(levels);
+    std::vector<size_t> off(levels);
+    size_t tot = 0;
+    for (uint32_t i = 0; i < levels; ++i) {
+        ws[i] = i == 0 ? width : ws[i - 1] / 2;
+        hs[i] = i == 0 ? height : hs[i - 1] / 2;
+        off[i] = tot;
+        tot += ((size_t)ws[i] * hs[i] + 255) & ~(size_t)255;
+    }
+    // blur3 weights (upsample.rs:47-50): exp(-1 / (2 * 0.25)) = 0x1.152aaap-3 (correctly rounded), normalised
+    const float e2 = 0x1.152aaap-3f;
+    const float sum = e2 + 1.0f + e2;
+    const float g[3] = {e2 / sum, 1.0f / sum, e2 / sum};
+    hipStream_t st = be->stream;
+    DevBuf<uint8_t> buf;
+    HIP_TRY(buf.reserve(tot));
+    hipError_t e = hipMemcpyAsync(buf.p, gray, (size_t)width * height, hipMemcpyHostToDevice, st);
+    for (uint32_t i = 1; e == hipSuccess && i < levels; ++i)
+        e = launch_pyramid_step(buf.p + off[i - 1], (int)ws[i - 1], (int)hs[i - 1], buf.p + off[i], g, st);
+    for (uint32_t i = 0; e == hipSuccess && i < levels; ++i)
+        e = hipMemcpyAsync(out[levels - 1 - i], buf.p + off[i], (size_t)ws[i] * hs[i], hipMemcpyDeviceToHost, st);
+    if (e == hipSuccess) e = hipStreamSynchronize(st);
+    buf.release();
+    if (e != hipSuccess) return fail(C3D_DEVICE, std::string("HIP: pyramid: ") + hipGetErrorString(e));
+    double kk[4] = {k[0], k[1], k[2], k[3]};
+    for (uint32_t i = 0; i < levels; ++i) {
+        if (i > 0) {
+            kk[0] *= 0.5;
+            kk[1] *= 0.5;
+            kk[2] = (kk[2] + 0.5) * 0.5 - 0.5;
+            kk[3] = (kk[3] + 0.5) * 0.5 - 0.5;
+        }
+        for (int j = 0; j < 4; ++j) k_out[4 * (levels - 1 - i) + j] = (float)kk[j];
     }
     return C3D_OK;
 }

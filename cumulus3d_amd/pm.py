@@ -26,7 +26,7 @@ _lock = threading.Lock()
 EXPORTS = [
     "c3d_pm_create", "c3d_pm_name", "c3d_pm_destroy", "c3d_pm_is_available", "c3d_pm_begin", "c3d_pm_run",
     "c3d_pm_evaluate", "c3d_pm_filter", "c3d_pm_upsample", "c3d_pm_median", "c3d_pm_end", "c3d_pm_last_error",
-    "c3d_pm_densify", "c3d_pm_pair_costs",
+    "c3d_pm_densify", "c3d_pm_pair_costs", "c3d_pm_num_levels", "c3d_pm_build_pyramid",
 ]
 
 
@@ -76,8 +76,13 @@ def load() -> C.CDLL:
                                      C.POINTER(abi.c3d_timing)]
         L.c3d_pm_pair_costs.argtypes = [C.c_void_p, C.c_uint32, C.c_uint32, C.c_uint32, C.c_uint32,
                                         C.POINTER(C.c_int32), C.POINTER(C.c_float), C.POINTER(C.c_float)]
+        L.c3d_pm_num_levels.argtypes = [C.c_uint32, C.c_uint32, C.c_uint32]
+        L.c3d_pm_num_levels.restype = C.c_uint32
+        L.c3d_pm_build_pyramid.argtypes = [C.c_void_p, C.POINTER(C.c_uint8), C.c_uint32, C.c_uint32,
+                                           C.POINTER(C.c_double), C.c_uint32, C.POINTER(C.POINTER(C.c_uint8)),
+                                           C.POINTER(C.c_float)]
         for n in ("c3d_pm_create", "c3d_pm_begin", "c3d_pm_run", "c3d_pm_evaluate", "c3d_pm_filter", "c3d_pm_upsample",
-                  "c3d_pm_median", "c3d_pm_densify", "c3d_pm_pair_costs", "c3d_pm_is_available"):
+                  "c3d_pm_median", "c3d_pm_densify", "c3d_pm_pair_costs", "c3d_pm_is_available", "c3d_pm_build_pyramid"):
             getattr(L, n).restype = C.c_int
         _libs["engine"] = L
         return L
@@ -86,6 +91,11 @@ def load() -> C.CDLL:
 def _check(L, rc: int):
     if rc != 0:
         raise PmError(rc, L.c3d_pm_last_error().decode(errors="replace"))
+
+
+def num_levels(min_side: int, max_levels: int, min_size: int) -> int:
+    """num_levels (crates/dense/src/upsample.rs:36-44)."""
+    return int(load().c3d_pm_num_levels(min_side, max_levels, min_size))
 
 
 def is_available() -> bool:
@@ -107,6 +117,18 @@ class HipPatchMatch:
 
     def begin(self, inp: abi.KernelInput) -> "Session":
         return Session(self, inp)
+
+    def build_pyramid(self, gray: np.ndarray, k: Sequence[float], levels: int) -> List[abi.LevelData]:
+        """Gray pyramid on the device (build_pyramid, crates/dense/src/upsample.rs:78-102): the
+        levels coarsest first, each with its intrinsics."""
+        gray = np.ascontiguousarray(gray, np.uint8)
+        h, w = gray.shape
+        outs = [np.zeros((h >> (levels - 1 - l), w >> (levels - 1 - l)), np.uint8) for l in range(levels)]
+        kk = (C.c_double * 4)(*[float(q) for q in k])
+        ko = (C.c_float * (4 * levels))()
+        _check(self.L, self.L.c3d_pm_build_pyramid(self.h, gray.ctypes.data_as(C.POINTER(C.c_uint8)), w, h, kk, levels,
+                                                   abi.ptr_array(outs, C.c_uint8), ko))
+        return [abi.LevelData(outs[l], [float(ko[4 * l + j]) for j in range(4)]) for l in range(levels)]
 
     def close(self):
         if self.h:
@@ -186,11 +208,12 @@ class Session:
             o.cost[...] = s.cost
         return outs
 
-    def densify(self, views: Sequence[int], photometric_iters: int, geometric_rounds: int, geometric_iters: int,
-                first_run_id: int = 1, download: bool = True):
-        """Device-resident single-level schedule (densify.rs:350-454). Returns (states with thresholded
-        depth, counts, raw depth, timing) or (None, None, None, timing) without download."""
-        sch = abi.c3d_schedule(photometric_iters, geometric_rounds, geometric_iters, first_run_id)
+    def densify(self, views: Sequence[int], schedule: abi.Schedule, download: bool = True):
+        """Device-resident compute_depth_maps schedule over every scale (densify.rs:350-454). Returns
+        (top-scale states with thresholded depth, counts, raw depth, timing), or (None, None, None,
+        timing) without download."""
+        sch = schedule.c()
+        top = self.inp.num_levels - 1
         tm = abi.c3d_timing()
         if not download:
             _check(self.L, self.L.c3d_pm_densify(self.h, abi.views_array(views), len(views), C.byref(sch), None, None, None,
@@ -198,7 +221,7 @@ class Session:
             return None, None, None, tm
         outs, counts, raws = [], [], []
         for v in views:
-            h, w = self.inp.level_shape(v, 0)
+            h, w = self.inp.level_shape(v, top)
             outs.append(abi.ViewState(w, h))
             counts.append(np.zeros((h, w), np.uint8))
             raws.append(np.zeros((h, w), np.float32))

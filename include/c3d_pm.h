@@ -234,18 +234,34 @@ const char* c3d_pm_last_error(void);
 
 /* ---------------------------------------------------------------- device-resident schedule */
 
-/* Single-level schedule of compute_depth_maps (crates/dense/src/densify.rs:350-454, level-0 slice)
- * executed with every state resident on the device: random-init photometric run, then
- * `geometric_rounds` rounds that each snapshot all views' depth on the device and run
- * `geometric_iters` geometric iterations, optional median filter, the filter read-out with a final
- * snapshot, and the count threshold max(1, min(min_num_consistent, nsrc)) (densify.rs:446-454).
+/* Schedule of compute_depth_maps (crates/dense/src/densify.rs:350-454) executed with every state
+ * resident on the device, over all `num_levels` scales of the input (0 = coarsest):
+ *   scale 0: random-init photometric run of `photometric_iters` iterations;
+ *   scale l > 0: joint bilateral upsampling of scale l-1 (`jbu_sigma_spatial`, `jbu_sigma_color`),
+ *     the detail restorer (densify.rs:366-409): evaluate the upsampled planes (c_init), run
+ *     `photometric_iters_upper` photometric iterations from them (or from a random init when
+ *     `restorer_random_init`), evaluate that result (c_photo), and keep the photometric plane where
+ *     the upsampled depth is not positive or c_init - c_photo > `restorer_threshold`;
+ *   every scale: `geometric_rounds` rounds that each snapshot all views' depth at that scale on the
+ *     device and run `geometric_iters` geometric iterations;
+ *   top scale: optional 5x5 median (filter.median_filter), the filter read-out with a final
+ *     snapshot, and the count threshold max(1, min(min_num_consistent, nsrc)) (densify.rs:446-454).
+ * Runs at scales > 0 use the upsampled plane as the weak-texture prior when pm.weak_texture_var > 0
+ * (densify.rs:366-372). LevelSchedule (params.rs:476-506) gives: fast profile 6 / 3 iterations, no
+ * restorer random init, 2 geometric rounds of 2; quality profile 7 / 6, random init, 2 rounds of 6.
  * There is no host round trip between runs (the reference copies states through the host after
  * every run, patchmatch.rs:785). */
 typedef struct c3d_schedule {
-    uint32_t photometric_iters;
-    uint32_t geometric_rounds;
+    uint32_t photometric_iters;       /* scale 0 */
+    uint32_t photometric_iters_upper; /* scales > 0 */
+    uint32_t geometric_rounds;        /* per scale */
     uint32_t geometric_iters;
     uint32_t first_run_id; /* run ids are first_run_id, first_run_id+1, ... (densify.rs:345-348) */
+    int32_t restorer_random_init;
+    float jbu_sigma_spatial;  /* DensifyOptions::jbu_sigma_spatial [1.0] (params.rs:423) */
+    float jbu_sigma_color;    /* DensifyOptions::jbu_sigma_color [0.04] */
+    float restorer_threshold; /* DensifyOptions::restorer_threshold [0.1] */
+    uint32_t pad0;
 } c3d_schedule;
 
 /* Per-stage device times of the last c3d_pm_densify call, milliseconds (hipEvent on the session
@@ -260,13 +276,29 @@ typedef struct c3d_timing {
     uint32_t pad0;
 } c3d_timing;
 
-/* Runs the schedule for `views` (all at level 0 of a single-level input). When `out` is not NULL
- * the final states are written: depth/normal zeroed where the threshold fails, raw cost kept; when
+/* Runs the schedule for `views`; `out`, `counts` and `raw_depth` are at the top scale
+ * (num_levels - 1). When `out` is not NULL the final states are written: depth/normal zeroed where the threshold fails, raw cost kept; when
  * `counts` is not NULL the filter counts are written. With both NULL the results stay on the
  * device (for throughput measurement). `timing` may be NULL. */
 int c3d_pm_densify(c3d_pm_session* session, const uint32_t* views, uint32_t num_views,
                    const c3d_schedule* schedule, c3d_view_state* out, uint8_t* const* counts,
                    float* const* raw_depth, c3d_timing* timing);
+
+/* ---------------------------------------------------------------- scene preparation */
+
+/* Number of scales (num_levels, crates/dense/src/upsample.rs:36-44): halve the short side while it
+ * stays >= min_size, at most max_levels, at least 1. Host only. */
+uint32_t c3d_pm_num_levels(uint32_t min_side, uint32_t max_levels, uint32_t min_size);
+
+/* Gray pyramid of one image, computed on the device of `backend` (build_pyramid,
+ * crates/dense/src/upsample.rs:78-102, called by densify.rs:293): each level is the previous one
+ * smoothed by the 3-tap Gaussian of blur3 (upsample.rs:45-75, edge clamp, rows then columns),
+ * 2x2-averaged and rounded to 8 bits; intrinsics f <- f/2, c <- (c + 0.5)/2 - 0.5 in double.
+ * Levels are in the reference's order, 0 = coarsest: out[levels-1] receives `gray` itself,
+ * level l is (width >> (levels-1-l)) x (height >> (levels-1-l)) pixels (successive floor
+ * halving), k_out[4l..4l+3] = fx, fy, cx, cy of level l. */
+int c3d_pm_build_pyramid(c3d_pm_backend* backend, const uint8_t* gray, uint32_t width, uint32_t height,
+                         const double k[4], uint32_t levels, uint8_t* const* out, float* k_out);
 
 /* ---------------------------------------------------------------- probes (no reference counterpart) */
 

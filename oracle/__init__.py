@@ -104,6 +104,20 @@ def load_library(path: str):
         return L
 
 
+def build_pyramid(gray: np.ndarray, k, levels: int):
+    """Gray pyramid (upsample.rs:78-102), coarsest first: list of (u8 image, [fx, fy, cx, cy])."""
+    gray = np.ascontiguousarray(gray, np.uint8)
+    h, w = gray.shape
+    outs = [np.zeros((h >> l, w >> l), np.uint8) for l in range(1, levels)]
+    ko = (C.c_float * (4 * levels))()
+    arr = (C.POINTER(C.c_uint8) * max(1, levels - 1))(*[o.ctypes.data_as(C.POINTER(C.c_uint8)) for o in outs])
+    lib().pmo_build_pyramid(gray.ctypes.data_as(C.POINTER(C.c_uint8)), w, h, (C.c_double * 4)(*[float(q) for q in k]),
+                            levels, arr, ko)
+    fine_first = [(gray, [float(ko[j]) for j in range(4)])] + [(outs[l - 1], [float(ko[4 * l + j]) for j in range(4)])
+                                                               for l in range(1, levels)]
+    return fine_first[::-1]
+
+
 class OracleError(RuntimeError):
     pass
 
@@ -169,13 +183,14 @@ class Oracle:
             o.cost[...] = s.cost
         return outs
 
-    def densify(self, views, photometric_iters, geometric_rounds, geometric_iters, first_run_id=1):
-        """Single-level schedule (densify.rs:350-454): returns (states with thresholded depth,
-        counts, raw depth)."""
-        sch = abi.c3d_schedule(photometric_iters, geometric_rounds, geometric_iters, first_run_id)
+    def densify(self, views, schedule: abi.Schedule):
+        """compute_depth_maps over every scale (densify.rs:350-454): returns top-scale (states with
+        thresholded depth, counts, raw depth)."""
+        sch = schedule.c()
+        top = self.inp.num_levels - 1
         outs, counts, raws = [], [], []
         for v in views:
-            h, w = self.inp.level_shape(v, 0)
+            h, w = self.inp.level_shape(v, top)
             outs.append(abi.ViewState(w, h))
             counts.append(np.zeros((h, w), np.uint8))
             raws.append(np.zeros((h, w), np.float32))

@@ -1436,105 +1436,177 @@ int pmo_median(pmo_ctx* c, uint32_t level, const uint32_t* views, uint32_t n, co
     return 0;
 }
 
-/* single-level compute_depth_maps slice (densify.rs:350-454) */
+/* ---------------------------------------------------------------- compute_depth_maps schedule */
+
+/* ViewState::new at `level` (kernel.rs:154-159): depth 0, normal 0, cost 2; prior zeroed when asked */
+static c3d_view_state* new_states(const pmo_ctx* c, uint32_t level, const uint32_t* views, uint32_t n, int prior) {
+    c3d_view_state* s = (c3d_view_state*)calloc(n, sizeof(c3d_view_state));
+    for (uint32_t i = 0; i < n; ++i) {
+        const lvl_t* L = &c->lv[views[i] * c->nl + level];
+        size_t np = (size_t)L->w * L->h;
+        s[i].width = (uint32_t)L->w;
+        s[i].height = (uint32_t)L->h;
+        s[i].depth = (float*)calloc(np, sizeof(float));
+        s[i].normal = (float*)calloc(3 * np, sizeof(float));
+        s[i].cost = (float*)malloc(np * sizeof(float));
+        for (size_t p = 0; p < np; ++p) s[i].cost[p] = 2.f;
+        s[i].prior = prior ? (float*)calloc(4 * np, sizeof(float)) : NULL;
+    }
+    return s;
+}
+
+static void free_states(c3d_view_state* s, uint32_t n) {
+    if (!s) return;
+    for (uint32_t i = 0; i < n; ++i) {
+        free(s[i].depth);
+        free(s[i].normal);
+        free(s[i].cost);
+        free(s[i].prior);
+    }
+    free(s);
+}
+
+static float** new_maps(const c3d_view_state* s, uint32_t n) {
+    float** m = (float**)calloc(n, sizeof(float*));
+    for (uint32_t i = 0; i < n; ++i) m[i] = (float*)malloc((size_t)s[i].width * s[i].height * sizeof(float));
+    return m;
+}
+
+static void free_maps(float** m, uint32_t n) {
+    if (!m) return;
+    for (uint32_t i = 0; i < n; ++i) free(m[i]);
+    free(m);
+}
+
+/* The snapshot closure of densify.rs:335-342: every active view's current depth at `level`. */
+static c3d_depth_snapshot take_snapshot(const pmo_ctx* c, const uint32_t* views, uint32_t n, const c3d_view_state* s,
+                                        uint32_t level, uint64_t id, float** copies, const float** maps) {
+    for (int v = 0; v < c->nv; ++v) maps[v] = NULL;
+    for (uint32_t i = 0; i < n; ++i) {
+        memcpy(copies[i], s[i].depth, (size_t)s[i].width * s[i].height * sizeof(float));
+        maps[views[i]] = copies[i];
+    }
+    c3d_depth_snapshot snap = {id, level, (uint32_t)c->nv, maps};
+    return snap;
+}
+
+/* compute_depth_maps over all scales (densify.rs:350-454). `out`, `counts`, `raw_depth` are at the
+ * top scale. */
 int pmo_densify(pmo_ctx* c, const uint32_t* views, uint32_t n, const c3d_schedule* sch, c3d_view_state* out,
                 uint8_t* const* counts, float* const* raw_depth) {
-    if (c->nl != 1) {
-        set_err("pmo_densify handles single-level inputs");
-        return C3D_INVALID_ARGUMENT;
-    }
-    int rc = check_states(c, 0, views, n, out);
+    const uint32_t top = (uint32_t)c->nl - 1;
+    int rc = check_states(c, top, views, n, out);
     if (rc) return rc;
-    for (uint32_t i = 0; i < n; ++i) {
-        size_t np = (size_t)out[i].width * out[i].height;
-        for (size_t p = 0; p < np; ++p) {
-            out[i].depth[p] = 0.f;
-            out[i].cost[p] = 2.f;
-            out[i].normal[3 * p] = out[i].normal[3 * p + 1] = out[i].normal[3 * p + 2] = 0.f;
-        }
-    }
     uint32_t run_id = sch->first_run_id;
-    c3d_run_params p = {0, 0, 1, sch->photometric_iters, run_id++, 0};
-    /* states passed to pmo_run must not carry priors here */
-    c3d_view_state* st = (c3d_view_state*)calloc(n, sizeof(c3d_view_state));
-    for (uint32_t i = 0; i < n; ++i) {
-        st[i] = out[i];
-        st[i].prior = NULL;
-    }
-    rc = pmo_run(c, views, n, st, &p, NULL);
     uint64_t snap_id = 0;
     const float** maps = (const float**)calloc((size_t)c->nv, sizeof(float*));
-    float** copies = (float**)calloc(n, sizeof(float*));
-    for (uint32_t r = 0; !rc && r < sch->geometric_rounds; ++r) {
-        for (uint32_t i = 0; i < n; ++i) {
-            size_t np = (size_t)st[i].width * st[i].height;
-            free(copies[i]);
-            copies[i] = (float*)malloc(np * sizeof(float));
-            memcpy(copies[i], st[i].depth, np * sizeof(float));
-            maps[views[i]] = copies[i];
-        }
-        c3d_depth_snapshot snap = {++snap_id, 0, (uint32_t)c->nv, maps};
-        c3d_run_params g = {0, 1, 0, sch->geometric_iters, run_id++, 0};
-        rc = pmo_run(c, views, n, st, &g, &snap);
-    }
-    if (!rc && c->in->filter.median_filter) {
-        /* median into temporaries, then back */
-        c3d_view_state* tmp = (c3d_view_state*)calloc(n, sizeof(c3d_view_state));
-        for (uint32_t i = 0; i < n; ++i) {
-            size_t np = (size_t)st[i].width * st[i].height;
-            tmp[i] = st[i];
-            tmp[i].depth = (float*)malloc(np * sizeof(float));
-            tmp[i].normal = (float*)malloc(3 * np * sizeof(float));
-            tmp[i].cost = NULL;
-        }
-        rc = pmo_median(c, 0, views, n, st, tmp);
-        for (uint32_t i = 0; i < n; ++i) {
-            size_t np = (size_t)st[i].width * st[i].height;
-            memcpy(st[i].depth, tmp[i].depth, np * sizeof(float));
-            memcpy(st[i].normal, tmp[i].normal, 3 * np * sizeof(float));
-            free(tmp[i].depth);
-            free(tmp[i].normal);
-        }
-        free(tmp);
-    }
-    uint8_t** cnt = (uint8_t**)calloc(n, sizeof(uint8_t*));
-    if (!rc) {
-        for (uint32_t i = 0; i < n; ++i) {
-            size_t np = (size_t)st[i].width * st[i].height;
-            free(copies[i]);
-            copies[i] = (float*)malloc(np * sizeof(float));
-            memcpy(copies[i], st[i].depth, np * sizeof(float));
-            maps[views[i]] = copies[i];
-            cnt[i] = counts ? counts[i] : (uint8_t*)malloc(np);
-        }
-        c3d_depth_snapshot snap = {++snap_id, 0, (uint32_t)c->nv, maps};
-        rc = pmo_filter(c, views, n, st, &snap, cnt);
-    }
-    if (!rc) {
-        for (uint32_t i = 0; i < n; ++i) {
-            const c3d_kernel_view* kv = &c->in->views[views[i]];
-            uint32_t need = c->in->filter.min_num_consistent < kv->num_sources ? c->in->filter.min_num_consistent
-                                                                               : kv->num_sources;
-            if (need < 1) need = 1;
-            if (need > 255) need = 255;
-            size_t np = (size_t)st[i].width * st[i].height;
-            if (raw_depth && raw_depth[i]) memcpy(raw_depth[i], st[i].depth, np * sizeof(float));
-            for (size_t p2 = 0; p2 < np; ++p2) {
-                if (!(st[i].depth[p2] > 0.f && cnt[i][p2] >= need)) {
-                    st[i].depth[p2] = 0.f;
-                    st[i].normal[3 * p2] = st[i].normal[3 * p2 + 1] = st[i].normal[3 * p2 + 2] = 0.f;
+    c3d_view_state* st = NULL;
+    for (uint32_t level = 0; !rc && level <= top; ++level) {
+        /* params.rs:476-506 via the schedule struct; use_prior densify.rs:366 */
+        const int use_prior = level > 0 && c->in->pm.weak_texture_var > 0.f;
+        if (level == 0) {
+            st = new_states(c, 0, views, n, 0);
+            c3d_run_params p = {0, 0, 1, sch->photometric_iters, run_id++, 0};
+            rc = pmo_run(c, views, n, st, &p, NULL);
+        } else {
+            /* detail restorer (densify.rs:366-409) */
+            c3d_view_state* up = new_states(c, level, views, n, use_prior);
+            rc = pmo_upsample(c, level, views, n, st, sch->jbu_sigma_spatial, sch->jbu_sigma_color, up);
+            free_states(st, n);
+            st = up;
+            if (rc) break;
+            if (use_prior)
+                for (uint32_t i = 0; i < n; ++i) {
+                    size_t np = (size_t)up[i].width * up[i].height;
+                    for (size_t q = 0; q < np; ++q) {
+                        up[i].prior[4 * q + 0] = up[i].normal[3 * q + 0];
+                        up[i].prior[4 * q + 1] = up[i].normal[3 * q + 1];
+                        up[i].prior[4 * q + 2] = up[i].normal[3 * q + 2];
+                        up[i].prior[4 * q + 3] = up[i].depth[q];
+                    }
+                }
+            float** c_init = new_maps(up, n);
+            float** c_photo = new_maps(up, n);
+            c3d_view_state* photo = new_states(c, level, views, n, use_prior);
+            for (uint32_t i = 0; i < n; ++i) {
+                size_t np = (size_t)up[i].width * up[i].height;
+                if (use_prior) memcpy(photo[i].prior, up[i].prior, 4 * np * sizeof(float));
+                if (!sch->restorer_random_init) {
+                    memcpy(photo[i].depth, up[i].depth, np * sizeof(float));
+                    memcpy(photo[i].normal, up[i].normal, 3 * np * sizeof(float));
+                    memcpy(photo[i].cost, up[i].cost, np * sizeof(float));
                 }
             }
+            rc = pmo_evaluate(c, level, views, n, up, 0, NULL, c_init);
+            c3d_run_params p = {level, 0, sch->restorer_random_init ? 1 : 0, sch->photometric_iters_upper, run_id++,
+                                use_prior};
+            if (!rc) rc = pmo_run(c, views, n, photo, &p, NULL);
+            if (!rc) rc = pmo_evaluate(c, level, views, n, photo, 0, NULL, c_photo);
+            const float xi = sch->restorer_threshold;
+            for (uint32_t i = 0; !rc && i < n; ++i) {
+                size_t np = (size_t)up[i].width * up[i].height;
+                for (size_t q = 0; q < np; ++q)
+                    if (up[i].depth[q] <= 0.f || c_init[i][q] - c_photo[i][q] > xi) {
+                        up[i].depth[q] = photo[i].depth[q];
+                        up[i].normal[3 * q + 0] = photo[i].normal[3 * q + 0];
+                        up[i].normal[3 * q + 1] = photo[i].normal[3 * q + 1];
+                        up[i].normal[3 * q + 2] = photo[i].normal[3 * q + 2];
+                    }
+            }
+            free_states(photo, n);
+            free_maps(c_init, n);
+            free_maps(c_photo, n);
+        }
+        float** copies = new_maps(st, n);
+        for (uint32_t r = 0; !rc && r < sch->geometric_rounds; ++r) {
+            c3d_depth_snapshot snap = take_snapshot(c, views, n, st, level, ++snap_id, copies, maps);
+            c3d_run_params g = {level, 1, 0, sch->geometric_iters, run_id++, use_prior};
+            rc = pmo_run(c, views, n, st, &g, &snap);
+        }
+        free_maps(copies, n);
+    }
+    /* priors dropped, median, read-out and threshold at the top scale (densify.rs:428-454) */
+    for (uint32_t i = 0; st && i < n; ++i) {
+        free(st[i].prior);
+        st[i].prior = NULL;
+    }
+    if (!rc && c->in->filter.median_filter) {
+        c3d_view_state* med = new_states(c, top, views, n, 0);
+        rc = pmo_median(c, top, views, n, st, med);
+        free_states(st, n);
+        st = med;
+    }
+    uint8_t** cnt = (uint8_t**)calloc(n, sizeof(uint8_t*));
+    float** copies = rc ? NULL : new_maps(st, n);
+    if (!rc) {
+        for (uint32_t i = 0; i < n; ++i)
+            cnt[i] = counts ? counts[i] : (uint8_t*)malloc((size_t)st[i].width * st[i].height);
+        c3d_depth_snapshot snap = take_snapshot(c, views, n, st, top, ++snap_id, copies, maps);
+        rc = pmo_filter(c, views, n, st, &snap, cnt);
+    }
+    for (uint32_t i = 0; !rc && i < n; ++i) {
+        const c3d_kernel_view* kv = &c->in->views[views[i]];
+        uint32_t need = c->in->filter.min_num_consistent < kv->num_sources ? c->in->filter.min_num_consistent
+                                                                           : kv->num_sources;
+        if (need < 1) need = 1;
+        if (need > 255) need = 255;
+        size_t np = (size_t)st[i].width * st[i].height;
+        if (raw_depth && raw_depth[i]) memcpy(raw_depth[i], st[i].depth, np * sizeof(float));
+        for (size_t q = 0; q < np; ++q) {
+            int keep = st[i].depth[q] > 0.f && cnt[i][q] >= need;
+            out[i].depth[q] = keep ? st[i].depth[q] : 0.f;
+            out[i].normal[3 * q + 0] = keep ? st[i].normal[3 * q + 0] : 0.f;
+            out[i].normal[3 * q + 1] = keep ? st[i].normal[3 * q + 1] : 0.f;
+            out[i].normal[3 * q + 2] = keep ? st[i].normal[3 * q + 2] : 0.f;
+            if (out[i].cost) out[i].cost[q] = st[i].cost[q];
         }
     }
-    for (uint32_t i = 0; i < n; ++i) {
-        free(copies[i]);
+    for (uint32_t i = 0; i < n; ++i)
         if (!counts && cnt[i]) free(cnt[i]);
-    }
-    free(copies);
     free(cnt);
+    free_maps(copies, n);
+    free_states(st, n);
     free(maps);
-    free(st);
     return rc;
 }
 

@@ -187,8 +187,8 @@ def test_densify_within_tolerance_of_ieee_divide(small):
     be = pm.HipPatchMatch()
     o = _oracle(inp, 0)
     with be.begin(inp) as s:
-        go, gc, graw, tm = s.densify([0, 1, 2], 5, 1, 2)
-    ro, rc, rraw = o.densify([0, 1, 2], 5, 1, 2)
+        go, gc, graw, tm = s.densify([0, 1, 2], abi.Schedule(photometric_iters=5, geometric_rounds=1, geometric_iters=2))
+    ro, rc, rraw = o.densify([0, 1, 2], abi.Schedule(photometric_iters=5, geometric_rounds=1, geometric_iters=2))
     assert tm.total_ms > 0 and tm.half_launches == 2 * (5 + 2)
     for v in range(3):
         valid, dfrac, nfrac = _agreement(graw[v], rraw[v], go[v].normal, ro[v].normal)
@@ -204,8 +204,8 @@ def test_densify_bit_exact(small):
     be = pm.HipPatchMatch()
     o = _oracle(inp, 1)
     with be.begin(inp) as s:
-        go, gc, graw, tm = s.densify([0, 1, 2], 3, 1, 2)
-    ro, rc, rraw = o.densify([0, 1, 2], 3, 1, 2)
+        go, gc, graw, tm = s.densify([0, 1, 2], abi.Schedule(photometric_iters=3, geometric_rounds=1, geometric_iters=2))
+    ro, rc, rraw = o.densify([0, 1, 2], abi.Schedule(photometric_iters=3, geometric_rounds=1, geometric_iters=2))
     for v in range(3):
         assert np.array_equal(graw[v].view(np.uint32), rraw[v].view(np.uint32))
         assert np.array_equal(gc[v], rc[v])

@@ -331,3 +331,23 @@ def test_ncc_fp64_brute_force(oracle_lib):
         assert abs(got - want) < 1e-6
         checked += 1
     assert checked > 20
+
+
+def test_pyramid_ramp_closed_form(oracle_lib):
+    """build_pyramid (upsample.rs:78-102) of a ramp: the symmetric 3-tap blur leaves a linear ramp
+    unchanged away from the clamped edges, so level 1 is the 2x2 mean 4i + 1 of the ramp 2x; the
+    same along columns. A transposed pass, a wrong averaging cell or a shifted cell fails it."""
+    import oracle
+
+    h, w = 40, 64
+    ramp_x = np.tile((2 * np.arange(w)).astype(np.uint8), (h, 1))
+    lv = oracle.build_pyramid(ramp_x, [100.0, 100.0, 31.5, 19.5], 3)
+    assert [l[0].shape for l in lv] == [(10, 16), (20, 32), (40, 64)]
+    l1 = lv[1][0].astype(int)
+    assert (l1[:, 1:-1] == 4 * np.arange(1, 31) + 1).all()
+    assert (l1 == l1[0]).all()  # rows identical
+    ramp_y = np.tile((2 * np.arange(h)).astype(np.uint8)[:, None], (1, w))
+    l1y = oracle.build_pyramid(ramp_y, [100.0, 100.0, 31.5, 19.5], 2)[0][0].astype(int)
+    assert (l1y[1:-1, :] == (4 * np.arange(1, 19) + 1)[:, None]).all()
+    # the intrinsics follow f/2, (c + 0.5)/2 - 0.5
+    assert lv[0][1] == [25.0, 25.0, 7.5, 4.5]
