@@ -30,7 +30,8 @@ class c3d_pm_options(C.Structure):
         ("tile_rows", C.c_int32),
         ("threads_per_pixel", C.c_int32),
         ("use_graph", C.c_int32),
-        ("reserved", C.c_int32 * 6),
+        ("any_radius", C.c_int32),
+        ("reserved", C.c_int32 * 5),
     ]
 
 

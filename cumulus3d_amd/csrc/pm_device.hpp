@@ -19,9 +19,12 @@
 #include <hip/hip_runtime.h>
 #include <stdint.h>
 
+#include <type_traits>
+
 namespace c3d {
 
 constexpr int MAXSRC = 32;
+constexpr int MAX_WR = 20;  // largest window radius (patchmatch.rs:292-294)
 constexpr float INF_ERR = 1e30f;
 
 struct LevelDesc {
@@ -83,6 +86,7 @@ struct KParams {
     int wv_stride;  // LDS floats per source in the per-thread view-weight table
     int ntiles;     // total workgroups of the launch
     int max_nsrc;   // largest source count of the batch (sizes the LDS view-weight table)
+    int wr;         // window radius (read by the kernels compiled for any radius)
 };
 
 // ---------------------------------------------------------------- work distribution
@@ -405,6 +409,127 @@ __device__ __forceinline__ float ncc_cost(const KParams& P, const RefWin<WR, WST
     return fminf(fmaxf(cc, 0.0f), 2.0f);
 }
 
+// ---------------------------------------------------------------- windows of any radius
+
+// Windows too large for register-resident weights (any radius up to 20, step 1 or 2; the kernels
+// instantiated with WR = 0 read the radius from KParams::wr): the bilateral weight of a sample is
+// recomputed where it is used, as the reference does for windows above 36 samples
+// (patchmatch.cu:47,153-165). Every operation and its order match the register-resident path, so
+// both give identical results for the same radius.
+struct RefWinDyn {
+    float ic, mu, W, iW, var;
+    int wr;
+};
+
+struct RefTileDyn {
+    const float* s;
+    int x0, y0, pitch;
+    __device__ __forceinline__ float at(int x, int y) const { return s[(y - y0) * pitch + (x - x0)]; }
+};
+
+template <int WR, int WSTEP>
+using RefWinT = typename std::conditional<(WR > 0), RefWin<WR, WSTEP>, RefWinDyn>::type;
+
+__device__ __forceinline__ float bilateral_weight(const KParams& P, float ic, float v, int dx, int dy) {
+    float d = ic - v;
+    float sp = (float)(dx * dx + dy * dy) * P.inv2s;
+    return c3d_expf(-fmaf(d * d, P.inv2c255, sp));
+}
+
+template <int WSTEP, class REF>
+__device__ void ref_stats_dyn(const KParams& P, const REF& ref, int x, int y, int wr, RefWinDyn& rw) {
+    const int wn = (2 * wr) / WSTEP + 1;
+    rw.ic = ref.at(x, y);
+    rw.wr = wr;
+    float W = 0.0f, S = 0.0f;
+    for (int a = 0; a < wn; ++a) {
+        const int dy = -wr + a * WSTEP;
+        for (int b = 0; b < wn; ++b) {
+            const int dx = -wr + b * WSTEP;
+            float v = ref.at(x + dx, y + dy);
+            float w = bilateral_weight(P, rw.ic, v, dx, dy);
+            W += w;
+            S = fmaf(w, v, S);
+        }
+    }
+    float mu = S / W;
+    float V = 0.0f;
+    for (int a = 0; a < wn; ++a) {
+        const int dy = -wr + a * WSTEP;
+        for (int b = 0; b < wn; ++b) {
+            const int dx = -wr + b * WSTEP;
+            float v = ref.at(x + dx, y + dy);
+            float dk = v - mu;
+            float wd = bilateral_weight(P, rw.ic, v, dx, dy) * dk;
+            V = fmaf(wd, dk, V);
+        }
+    }
+    rw.mu = mu;
+    rw.W = W;
+    rw.iW = 1.0f / W;
+    rw.var = V / W;
+}
+
+template <int WSTEP, class REF>
+__device__ float ncc_cost_dyn(const KParams& P, const RefWinDyn& rw, const REF& ref, const LevelDesc& S, const float* H, int x,
+                              int y) {
+    float z = H[8];
+    if (!(z > 1e-8f)) return 2.0f;
+    float iz = 1.0f / z;
+    float u = H[2] * iz;
+    float v = H[5] * iz;
+    if (!(u >= 0.f && v >= 0.f && u <= (float)(S.w - 1) && v <= (float)(S.h - 1))) return 2.0f;
+    if (rw.var < P.vmin255) return 2.0f;
+    const uint32_t* qbase = P.quad + (S.qoff - 2LL * S.qpitch - 2);
+    const float qp4 = 4.0f * (float)S.qpitch, koff4 = 2.0f * qp4 + 8.0f;
+    const float w1 = (float)S.w, h1 = (float)S.h;
+    const int wr = rw.wr, wn = (2 * wr) / WSTEP + 1;
+    float Ss = 0.f, SSs = 0.f, IS = 0.f;
+    for (int a = 0; a < wn; ++a) {
+        const int dyi = -wr + a * WSTEP;
+        const float oy = (float)dyi;
+        const float rz = fmaf(H[7], oy, H[8]);
+        const float ru = fmaf(H[1], oy, H[2]);
+        const float rv = fmaf(H[4], oy, H[5]);
+#pragma unroll 2
+        for (int b = 0; b < wn; ++b) {
+            const int dxi = -wr + b * WSTEP;
+            const float ox = (float)dxi;
+            const float zz = fmaf(H[6], ox, rz);
+            const float r = rcp_sample(zz);
+            float sv = quad_sample(qbase, qp4, koff4, w1, h1, fmaf(H[0], ox, ru) * r, fmaf(H[3], ox, rv) * r);
+            sv = zz > 1e-8f ? sv : 0.0f;
+            const float iv = ref.at(x + dxi, y + dyi);
+            const float sp = sv - rw.ic;
+            const float ws = bilateral_weight(P, rw.ic, iv, dxi, dyi) * sp;
+            Ss += ws;
+            SSs = fmaf(ws, sp, SSs);
+            IS = fmaf(ws, iv, IS);
+        }
+    }
+    const float iW = rw.iW;
+    float mus = Ss * iW;
+    float vars = fmaf(-mus, mus, SSs * iW);
+    if (vars < P.vmin255) return 2.0f;
+    float cov = fmaf(-rw.mu, mus, IS * iW);
+    float cc = 1.0f - cov / sqrtf(rw.var * vars);
+    return fminf(fmaxf(cc, 0.0f), 2.0f);
+}
+
+// Dispatch between the register-resident window (WR > 0) and the window of any radius (WR = 0).
+template <int WR, int WSTEP, class REF>
+__device__ __forceinline__ void ref_stats_any(const KParams& P, const REF& ref, int x, int y, int wr, RefWinT<WR, WSTEP>& rw) {
+    if constexpr (WR > 0) ref_stats<WR, WSTEP>(P, ref, x, y, rw);
+    else ref_stats_dyn<WSTEP>(P, ref, x, y, wr, rw);
+}
+
+template <int WR, int WSTEP, class REF>
+__device__ __forceinline__ float ncc_cost_any(const KParams& P, const RefWinT<WR, WSTEP>& rw, const REF& ref, const LevelDesc& S,
+                                              const float* H, int x, int y) {
+    if constexpr (WR > 0) return ncc_cost<WR, WSTEP>(P, rw, ref, S, H, x, y);
+    else return ncc_cost_dyn<WSTEP>(P, rw, ref, S, H, x, y);
+}
+
 // Forward-backward reprojection error through the source depth snapshot (patchmatch.cu:292-314).
 __device__ __forceinline__ float geo_error(const KParams& P, const LevelDesc& R, const LevelDesc& S, const PairRec& pr, int x, int y,
                                            float d) {
@@ -432,12 +557,13 @@ __device__ __forceinline__ float geo_error(const KParams& P, const LevelDesc& R,
 
 // (photometric cost, truncated geometric error) of one hypothesis x one source (patchmatch.cu:316-324)
 template <int WR, int WSTEP, class REF>
-__device__ __forceinline__ float pair_cost(const KParams& P, const RefWin<WR, WSTEP>& rw, const REF& ref, const LevelDesc& R,
+__device__ __forceinline__ float pair_cost(const KParams& P, const RefWinT<WR, WSTEP>& rw, const REF& ref, const LevelDesc& R,
                                            const LevelDesc& S, const PairRec& pr, const LevelPair& lp, int x, int y, float4 h,
                                            float* geo) {
     float H[9];
     float c = 2.0f;
-    if (homography(R, lp, 1.0f / R.fx, 1.0f / R.fy, (float)x, (float)y, h.w, xyz(h), H)) c = ncc_cost<WR, WSTEP>(P, rw, ref, S, H, x, y);
+    if (homography(R, lp, 1.0f / R.fx, 1.0f / R.fy, (float)x, (float)y, h.w, xyz(h), H))
+        c = ncc_cost_any<WR, WSTEP>(P, rw, ref, S, H, x, y);
     if (P.geometric) *geo = fminf(geo_error(P, R, S, pr, x, y, h.w), P.gmax);
     return c;
 }
@@ -486,10 +612,9 @@ __device__ __forceinline__ f3 perturb_normal(unsigned long long& st, f3 n, f3 ra
     return n;
 }
 
-// triangulation x incidence x resolution prior (patchmatch.cu:388-422)
-template <int WR>
+// triangulation x incidence x resolution prior (patchmatch.cu:388-422); r = window radius
 __device__ __forceinline__ float view_prior(const KParams& P, const LevelDesc& R, const PairRec& pr, const float* H, bool hok, int x,
-                                            int y, float4 h) {
+                                            int y, float4 h, float r) {
     if (!hok) return 0.f;
     f3 ray = pixel_ray(R, x, y);
     f3 X = mk3(ray.x * h.w, ray.y * h.w, h.w);
@@ -505,7 +630,6 @@ __device__ __forceinline__ float view_prior(const KParams& P, const LevelDesc& R
     float cphi = dot3(S, xyz(h)) / ls;
     float xi = 1.f - fmaxf(0.f, cphi);
     float inc = c3d_expf(-xi * xi * P.inv2inc);
-    const float r = (float)WR;
     const float cxs[4] = {-r, -r, r, r};
     const float cys[4] = {-r, r, r, -r};
     float px[4], py[4];

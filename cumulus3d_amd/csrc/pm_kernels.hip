@@ -41,15 +41,25 @@ __device__ __forceinline__ float4 pick4(const float4 (&a)[N], int i) {
     return r;
 }
 
+// Reference tile accessor: compile-time pitch for the fixed windows, runtime pitch for WR = 0.
+template <int WR, int TW>
+using TileRefT = typename std::conditional<(WR > 0), RefTile<TW>, RefTileDyn>::type;
+
+template <int WR, int TW>
+__device__ __forceinline__ TileRefT<WR, TW> tile_ref(const float* tile, int x0, int y0, int pitch) {
+    if constexpr (WR > 0) return RefTile<TW>{tile, x0, y0};
+    else return RefTileDyn{tile, x0, y0, pitch};
+}
+
 // (photometric cost, truncated geometric error) of one hypothesis x one source with the pixel's
 // reciprocal focal lengths hoisted (patchmatch.cu:316-324)
 template <int WR, int WSTEP, bool GEO, class REF>
-__device__ __forceinline__ float pair_cost_h(const KParams& P, const RefWin<WR, WSTEP>& rw, const REF& ref, const LevelDesc& R,
+__device__ __forceinline__ float pair_cost_h(const KParams& P, const RefWinT<WR, WSTEP>& rw, const REF& ref, const LevelDesc& R,
                                              const LevelDesc& S, const PairRec& pr, const LevelPair& lp, float ifx, float ify, int x,
                                              int y, float4 h, float& geo) {
     float H[9];
     float c = 2.0f;
-    if (homography(R, lp, ifx, ify, (float)x, (float)y, h.w, xyz(h), H)) c = ncc_cost<WR, WSTEP>(P, rw, ref, S, H, x, y);
+    if (homography(R, lp, ifx, ify, (float)x, (float)y, h.w, xyz(h), H)) c = ncc_cost_any<WR, WSTEP>(P, rw, ref, S, H, x, y);
     geo = GEO ? fminf(geo_error(P, R, S, pr, x, y, h.w), P.gmax) : 0.f;
     return c;
 }
@@ -60,10 +70,15 @@ template <int WR, int WSTEP, bool GEO>
 __global__ __launch_bounds__(256) __attribute__((amdgpu_waves_per_eu(2, 2))) void k_half(KParams P) {
     constexpr int TW = HALF_SPAN + 2 * WR;
     constexpr int TH = HALF_ROWS + 2 * WR;
-    __shared__ float tile[TW * TH];
+    __shared__ float tile_s[WR > 0 ? TW * TH : 1];
     // per-thread hypothesis table: the 9 candidates of step 2, then the refinement candidates
     __shared__ float4 hyp_lds[9 * 256];
-    extern __shared__ float wv_lds[];
+    // dynamic LDS: [reference tile (WR = 0 only)] [per-thread view weights]
+    extern __shared__ float dyn_lds[];
+    const int wr = WR > 0 ? WR : P.wr;
+    const int twd = HALF_SPAN + 2 * wr, thd = HALF_ROWS + 2 * wr;
+    float* tile = WR > 0 ? tile_s : dyn_lds;
+    float* wv_lds = WR > 0 ? dyn_lds : dyn_lds + twd * thd;
 
     BatchItem it;
     int px0, py0;
@@ -72,9 +87,9 @@ __global__ __launch_bounds__(256) __attribute__((amdgpu_waves_per_eu(2, 2))) voi
     const int w = it.w, h = it.h;
     {
         const uint8_t* g = P.gray + R.goff;
-        const int tx0 = px0 - WR, ty0 = py0 - WR;
-        for (int i = threadIdx.x; i < TW * TH; i += 256) {
-            int ty = i / TW, tx = i - ty * TW;
+        const int tx0 = px0 - wr, ty0 = py0 - wr;
+        for (int i = threadIdx.x; i < twd * thd; i += 256) {
+            int ty = i / twd, tx = i - ty * twd;
             int gx = tx0 + tx, gy = ty0 + ty;
             tile[i] = (gx >= 0 && gy >= 0 && gx < w && gy < h) ? (float)g[(size_t)gy * w + gx] : 0.0f;
         }
@@ -85,7 +100,7 @@ __global__ __launch_bounds__(256) __attribute__((amdgpu_waves_per_eu(2, 2))) voi
     const int x = px0 + 2 * lane + ((y + P.color) & 1);
     if (x >= w || y >= h) return;
 
-    RefTile<TW> ref{tile, px0 - WR, py0 - WR};
+    const auto ref = tile_ref<WR, TW>(tile, px0 - wr, py0 - wr, twd);
     const ViewDesc vd = P.views[it.view];
     const int nsrc = vd.nsrc;
     const PairRec* prs = P.pairs + (size_t)it.view * MAXSRC;
@@ -142,8 +157,8 @@ __global__ __launch_bounds__(256) __attribute__((amdgpu_waves_per_eu(2, 2))) voi
     }
 
     // 2. view selection and aggregation (patchmatch.cu:513-594)
-    RefWin<WR, WSTEP> rw;
-    ref_stats<WR, WSTEP>(P, ref, x, y, rw);
+    RefWinT<WR, WSTEP> rw;
+    ref_stats_any<WR, WSTEP>(P, ref, x, y, wr, rw);
     const int vprev = P.bview[base + pix];
     float num[9], numf[9];
 #pragma unroll
@@ -159,7 +174,7 @@ __global__ __launch_bounds__(256) __attribute__((amdgpu_waves_per_eu(2, 2))) voi
         {
             float H[9];
             bool ok = homography(R, lp, ifx, ify, (float)x, (float)y, hyp0.w, xyz(hyp0), H);
-            prior = view_prior<WR>(P, R, pr, H, ok, x, y, hyp0);
+            prior = view_prior(P, R, pr, H, ok, x, y, hyp0, (float)wr);
         }
         wv[j * P.wv_stride] = 0.f;
         if (!(prior > 0.f)) continue;
@@ -283,10 +298,9 @@ __global__ __launch_bounds__(256) __attribute__((amdgpu_waves_per_eu(2, 2))) voi
 }
 
 // Stages the reference rows of a 64 x 4 pixel tile (plus the window margin) in LDS.
-template <int WR>
-__device__ __forceinline__ void stage_pix_tile(float* tile, const uint8_t* g, int w, int h, int px0, int py0) {
-    constexpr int TW = PIX_TW + 2 * WR, TH = PIX_TH + 2 * WR;
-    const int tx0 = px0 - WR, ty0 = py0 - WR;
+__device__ __forceinline__ void stage_pix_tile(float* tile, const uint8_t* g, int w, int h, int px0, int py0, int wr) {
+    const int TW = PIX_TW + 2 * wr, TH = PIX_TH + 2 * wr;
+    const int tx0 = px0 - wr, ty0 = py0 - wr;
     for (int i = threadIdx.x; i < TW * TH; i += 256) {
         int ty = i / TW, tx = i - ty * TW;
         int gx = tx0 + tx, gy = ty0 + ty;
@@ -298,12 +312,15 @@ __device__ __forceinline__ void stage_pix_tile(float* tile, const uint8_t* g, in
 template <int WR, int WSTEP>
 __global__ __launch_bounds__(256) void k_eval(KParams P, int mode) {
     constexpr int TW = PIX_TW + 2 * WR;
-    __shared__ float tile[TW * (PIX_TH + 2 * WR)];
+    __shared__ float tile_s[WR > 0 ? TW * (PIX_TH + 2 * WR) : 1];
+    extern __shared__ float dyn_lds[];
+    const int wr = WR > 0 ? WR : P.wr;
+    float* tile = WR > 0 ? tile_s : dyn_lds;
     BatchItem it;
     int px0, py0;
     if (!locate_tile(P, xcd_tile(blockIdx.x, P.ntiles), PIX_TW, PIX_TH, it, px0, py0)) return;
     const LevelDesc R = P.lv[it.view * P.nlevels + P.level];
-    stage_pix_tile<WR>(tile, P.gray + R.goff, it.w, it.h, px0, py0);
+    stage_pix_tile(tile, P.gray + R.goff, it.w, it.h, px0, py0, wr);
     __syncthreads();
     const int x = px0 + (threadIdx.x & 63), y = py0 + (threadIdx.x >> 6);
     if (x >= it.w || y >= it.h) return;
@@ -323,9 +340,9 @@ __global__ __launch_bounds__(256) void k_eval(KParams P, int mode) {
     }
     float c = 2.f;
     if (h.w > 0.f) {
-        RefTile<TW> ref{tile, px0 - WR, py0 - WR};
-        RefWin<WR, WSTEP> rw;
-        ref_stats<WR, WSTEP>(P, ref, x, y, rw);
+        const auto ref = tile_ref<WR, TW>(tile, px0 - wr, py0 - wr, PIX_TW + 2 * wr);
+        RefWinT<WR, WSTEP> rw;
+        ref_stats_any<WR, WSTEP>(P, ref, x, y, wr, rw);
         // top-K mean (patchmatch.cu:326-345)
         float b0 = 1e30f, b1 = 1e30f, b2 = 1e30f;
 #pragma nounroll
@@ -360,12 +377,15 @@ __global__ __launch_bounds__(256) void k_eval(KParams P, int mode) {
 template <int WR, int WSTEP>
 __global__ __launch_bounds__(256) void k_filter(KParams P) {
     constexpr int TW = PIX_TW + 2 * WR;
-    __shared__ float tile[TW * (PIX_TH + 2 * WR)];
+    __shared__ float tile_s[WR > 0 ? TW * (PIX_TH + 2 * WR) : 1];
+    extern __shared__ float dyn_lds[];
+    const int wr = WR > 0 ? WR : P.wr;
+    float* tile = WR > 0 ? tile_s : dyn_lds;
     BatchItem it;
     int px0, py0;
     if (!locate_tile(P, xcd_tile(blockIdx.x, P.ntiles), PIX_TW, PIX_TH, it, px0, py0)) return;
     const LevelDesc R = P.lv[it.view * P.nlevels + P.level];
-    stage_pix_tile<WR>(tile, P.gray + R.goff, it.w, it.h, px0, py0);
+    stage_pix_tile(tile, P.gray + R.goff, it.w, it.h, px0, py0, wr);
     __syncthreads();
     const int x = px0 + (threadIdx.x & 63), y = py0 + (threadIdx.x >> 6);
     if (x >= it.w || y >= it.h) return;
@@ -377,9 +397,9 @@ __global__ __launch_bounds__(256) void k_filter(KParams P) {
     float4 h = P.plane[base + pix];
     int count = 0;
     if (h.w > 0.f) {
-        RefTile<TW> ref{tile, px0 - WR, py0 - WR};
-        RefWin<WR, WSTEP> rw;
-        ref_stats<WR, WSTEP>(P, ref, x, y, rw);
+        const auto ref = tile_ref<WR, TW>(tile, px0 - wr, py0 - wr, PIX_TW + 2 * wr);
+        RefWinT<WR, WSTEP> rw;
+        ref_stats_any<WR, WSTEP>(P, ref, x, y, wr, rw);
         f3 ray = pixel_ray(R, x, y);
         f3 X = mk3(ray.x * h.w, ray.y * h.w, h.w);
         float lx = sqrtf(dot3(X, X));
@@ -396,7 +416,7 @@ __global__ __launch_bounds__(256) void k_filter(KParams P) {
             if (cphi <= 0.f) continue;
             float H[9];
             if (!homography(R, lps[j], 1.0f / R.fx, 1.0f / R.fy, (float)x, (float)y, h.w, xyz(h), H)) continue;
-            float m = ncc_cost<WR, WSTEP>(P, rw, ref, S, H, x, y);
+            float m = ncc_cost_any<WR, WSTEP>(P, rw, ref, S, H, x, y);
             float e = P.ncc_norm * c3d_expf(-m * m * P.inv2ncc);
             if (e / (e + 0.5f) < P.q_min) continue;
             if (geo_error(P, R, S, pr, x, y, h.w) > P.filter_gmax) continue;
@@ -418,12 +438,13 @@ __global__ __launch_bounds__(256) void k_pair_costs(KParams P, int view, int slo
     const LevelDesc S = P.lv[pr.src * P.nlevels + P.level];
     int x = xy[2 * i], y = xy[2 * i + 1];
     RefGlobal g{P.gray + R.goff, R.w, R.h};
-    RefWin<WR, WSTEP> rw;
-    ref_stats<WR, WSTEP>(P, g, x, y, rw);
+    const int wr = WR > 0 ? WR : P.wr;
+    RefWinT<WR, WSTEP> rw;
+    ref_stats_any<WR, WSTEP>(P, g, x, y, wr, rw);
     float4 h = planes[i];
     float H[9];
     float c = 2.f;
-    if (homography(R, lp, 1.0f / R.fx, 1.0f / R.fy, (float)x, (float)y, h.w, xyz(h), H)) c = ncc_cost<WR, WSTEP>(P, rw, g, S, H, x, y);
+    if (homography(R, lp, 1.0f / R.fx, 1.0f / R.fy, (float)x, (float)y, h.w, xyz(h), H)) c = ncc_cost_any<WR, WSTEP>(P, rw, g, S, H, x, y);
     out[i] = c;
 }
 
@@ -631,21 +652,26 @@ int pix_tiles(const BatchItem* items, int n) {
     return t;
 }
 
+// dynamic LDS bytes of the any-radius (WR = 0) reference tiles
+static size_t half_tile_bytes(int wr) { return (size_t)(HALF_SPAN + 2 * wr) * (HALF_ROWS + 2 * wr) * sizeof(float); }
+static size_t pix_tile_bytes(int wr) { return (size_t)(PIX_TW + 2 * wr) * (PIX_TH + 2 * wr) * sizeof(float); }
+
 template <int WR, int WSTEP>
 static hipError_t launch_half(const KParams& P, hipStream_t s) {
     size_t lds = (size_t)P.wv_stride * (P.max_nsrc > 0 ? P.max_nsrc : 1) * sizeof(float);
+    if (WR == 0) lds += half_tile_bytes(P.wr);
     if (P.geometric) hipLaunchKernelGGL((k_half<WR, WSTEP, true>), dim3(P.ntiles), dim3(256), lds, s, P);
     else hipLaunchKernelGGL((k_half<WR, WSTEP, false>), dim3(P.ntiles), dim3(256), lds, s, P);
     return hipGetLastError();
 }
 template <int WR, int WSTEP>
 static hipError_t launch_eval(const KParams& P, int mode, hipStream_t s) {
-    hipLaunchKernelGGL((k_eval<WR, WSTEP>), dim3(P.ntiles), dim3(256), 0, s, P, mode);
+    hipLaunchKernelGGL((k_eval<WR, WSTEP>), dim3(P.ntiles), dim3(256), WR == 0 ? pix_tile_bytes(P.wr) : 0, s, P, mode);
     return hipGetLastError();
 }
 template <int WR, int WSTEP>
 static hipError_t launch_filter(const KParams& P, hipStream_t s) {
-    hipLaunchKernelGGL((k_filter<WR, WSTEP>), dim3(P.ntiles), dim3(256), 0, s, P);
+    hipLaunchKernelGGL((k_filter<WR, WSTEP>), dim3(P.ntiles), dim3(256), WR == 0 ? pix_tile_bytes(P.wr) : 0, s, P);
     return hipGetLastError();
 }
 template <int WR, int WSTEP>
@@ -667,14 +693,18 @@ static constexpr KernelSet make_set() {
 
 // Window variants compiled into the library (radius, step): the fast profile (5, 2) and the
 // variants of the reference's window report (patchmatch_gpu.rs:164) whose window holds at most
-// 49 samples, so that the bilateral weights stay in registers.
+// 49 samples, with the bilateral weights in registers; every other radius up to 20 (the quality
+// profile's (5, 1) among them) runs the any-radius kernels (wr = 0 below) of its step.
 static const KernelSet kSets[] = {make_set<5, 2>(), make_set<4, 2>(), make_set<3, 2>(), make_set<2, 2>(), make_set<3, 1>(),
                                   make_set<2, 1>(), make_set<1, 1>()};
+static const KernelSet kAny[] = {make_set<0, 1>(), make_set<0, 2>()};
 
-const KernelSet* kernel_set(int wr, int wstep) {
-    for (const KernelSet& k : kSets)
-        if (k.wr == wr && k.wstep == wstep) return &k;
-    return nullptr;
+const KernelSet* kernel_set(int wr, int wstep, bool any_radius) {
+    if (wr < 1 || wr > MAX_WR || wstep < 1 || wstep > 2) return nullptr;
+    if (!any_radius)
+        for (const KernelSet& k : kSets)
+            if (k.wr == wr && k.wstep == wstep) return &k;
+    return &kAny[wstep - 1];
 }
 
 hipError_t launch_upsample(const KParams& P, hipStream_t s) {

@@ -16,8 +16,10 @@ struct KernelSet {
     const void* (*half_symbol)();
 };
 
-// nullptr when the library has no kernels for this window
-const KernelSet* kernel_set(int wr, int wstep);
+// Kernels of a window (radius 1..MAX_WR, step 1 or 2): the fixed-radius build when compiled and
+// `any_radius` is false, else the any-radius kernels of the step (KernelSet::wr == 0); nullptr
+// outside those bounds.
+const KernelSet* kernel_set(int wr, int wstep, bool any_radius);
 
 // workgroup counts of the two tilings for a batch
 int half_tiles(const BatchItem* items, int n);

@@ -454,7 +454,7 @@ int c3d_pm_begin(c3d_pm_backend* be, const c3d_kernel_input* in, c3d_pm_session*
     if (pm.window_radius < 1 || pm.window_radius > 20 || pm.window_step < 1 || pm.window_step > 2)
         return fail(C3D_INVALID_ARGUMENT, "window radius " + std::to_string(pm.window_radius) + " (1..=20), step " +
                                               std::to_string(pm.window_step) + " (1|2)");
-    const KernelSet* ks = kernel_set(pm.window_radius, pm.window_step);
+    const KernelSet* ks = kernel_set(pm.window_radius, pm.window_step, be->opts.any_radius != 0);
     if (!ks)
         return fail(C3D_INVALID_ARGUMENT, "window radius " + std::to_string(pm.window_radius) + " step " + std::to_string(pm.window_step) +
                                               " has no compiled kernels in this build");
@@ -564,6 +564,7 @@ int c3d_pm_begin(c3d_pm_backend* be, const c3d_kernel_input* in, c3d_pm_session*
     float sc = pm.sigma_spatial > 0.0f ? pm.sigma_spatial : (float)pm.window_radius;
     const c3d_filter_params& fl = in->filter;
     k.seed = in->seed;
+    k.wr = pm.window_radius;
     k.nlevels = nl;
     k.topk = (int)pm.top_k;
     k.tau_bad = pm.tau1;

@@ -58,7 +58,9 @@ typedef struct c3d_pm_options {
     int32_t tile_rows;        /* rows of active pixels per workgroup; 0 = tuned default */
     int32_t threads_per_pixel;/* lanes cooperating on one pixel (1, 2 or 4); 0 = tuned default */
     int32_t use_graph;        /* 1 = capture each run's iteration loop in a hipGraph; 0 = plain launches */
-    int32_t reserved[6];
+    int32_t any_radius;       /* 1 = always the any-radius kernels (weights recomputed per sample), even
+                                 for the windows with a register-resident build; 0 = fastest available */
+    int32_t reserved[5];
 } c3d_pm_options;
 
 /* One view image at one scale (LevelImage, kernel.rs:81-91). k = fx, fy, cx, cy. */

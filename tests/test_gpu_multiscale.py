@@ -9,7 +9,7 @@ Bars:
     quality-profile variant with the restorer's random init, the weak-texture prior and the median
     filter switched on;
   * on the reference's own ground-truth scene (SynthConfig::default(): 3 x 3 cameras, 384 x 288,
-    3 scales) the bars of crates/cuda/tests/patchmatch_gpu.rs:96-98 hold for the fast profile
+    3 scales) the bars of crates/cuda/tests/patchmatch_gpu.rs:96-98 hold for both profiles
     (depth median relative error < 0.005, normal median < 3 deg, valid > 0.5), and a second run is
     bit-identical (patchmatch_gpu.rs:112-117).
 """
@@ -124,7 +124,7 @@ def synth():
     return sc, scenes.select_sources(sc), scenes.depth_ranges(sc)
 
 
-@pytest.mark.parametrize("profile", ["fast"])
+@pytest.mark.parametrize("profile", ["fast", "quality"])
 def test_reference_ground_truth_bars(synth, profile):
     sc, src, ranges = synth
     be = pm.HipPatchMatch()
