@@ -351,3 +351,86 @@ def test_pyramid_ramp_closed_form(oracle_lib):
     assert (l1y[1:-1, :] == (4 * np.arange(1, 19) + 1)[:, None]).all()
     # the intrinsics follow f/2, (c + 0.5)/2 - 0.5
     assert lv[0][1] == [25.0, 25.0, 7.5, 4.5]
+
+
+def _two_level_input(low_gray, high_gray, lk, hk):
+    v = abi.ViewData(key=1, levels=[abi.LevelData(low_gray, lk), abi.LevelData(high_gray, hk)], sources=[], pairs=[],
+                     depth_min=1.0, depth_max=10.0)
+    pmp, fl = abi.fast_profile()
+    return abi.KernelInput([v], pmp, fl)
+
+
+def _ray(k, x, y):
+    return np.array([(x - k[2]) / k[0], (y - k[3]) / k[1], 1.0])
+
+
+def test_upsample_reproduces_plane(oracle_lib):
+    """upsample.rs:278-307: a tilted plane upsampled from 16 x 12 to 32 x 24 matches the plane
+    along every high-resolution ray (1e-5), and the median filter keeps it."""
+    import oracle
+
+    hk = [40.0, 40.0, 15.5, 11.5]
+    lk = [20.0, 20.0, (15.5 + 0.5) * 0.5 - 0.5, (11.5 + 0.5) * 0.5 - 0.5]
+    n = np.array([0.3, -0.2, -1.0])
+    n /= np.linalg.norm(n)
+    hit = lambda r: -5.0 / (n @ r)
+    o = oracle.Oracle(_two_level_input(np.full((12, 16), 50, np.uint8), np.full((24, 32), 50, np.uint8), lk, hk), threads=1)
+    low = abi.ViewState(16, 12)
+    for y in range(12):
+        for x in range(16):
+            low.depth[y, x] = hit(_ray(lk, x, y))
+            low.normal[y, x] = n
+    up = o.upsample(1, [0], [low], 1.0, 0.04)[0]
+    for y in range(24):
+        for x in range(32):
+            t = hit(_ray(hk, x, y))
+            assert abs((up.depth[y, x] - t) / t) < 1e-5
+    m = o.median_filter(1, [0], [up])[0]
+    assert abs((m.depth.reshape(-1)[100] - up.depth.reshape(-1)[100]) / up.depth.reshape(-1)[100]) < 1e-5
+
+
+def test_upsample_weights_and_median_fp64(oracle_lib):
+    """Joint bilateral weights (patchmatch.cu:741-800) and the 5 x 5 lower median
+    (patchmatch.cu:803-849) against an independent FP64 evaluation: fronto-parallel planes transfer
+    their depth unchanged, so the upsampled depth is the weighted mean of the 5 x 5 low-resolution
+    depths around round((x - 0.5) / 2) with weights exp(-ds / 2 s_s^2 - (I_p - I_q)^2 / 2 s_c^2),
+    and the median filter output is the lower median of the 5 x 5 depths."""
+    import oracle
+
+    rng = np.random.default_rng(5)
+    lo_g = rng.integers(0, 256, (12, 16)).astype(np.uint8)
+    hi_g = rng.integers(0, 256, (24, 32)).astype(np.uint8)
+    hk = [40.0, 40.0, 15.5, 11.5]
+    lk = [20.0, 20.0, 7.5, 5.5]
+    o = oracle.Oracle(_two_level_input(lo_g, hi_g, lk, hk), threads=1)
+    low = abi.ViewState(16, 12)
+    low.depth[...] = rng.uniform(5.0, 9.0, (12, 16)).astype(np.float32)
+    low.normal[...] = np.array([0.0, 0.0, -1.0], np.float32)
+    ss, sc = 1.0, 0.3
+    up = o.upsample(1, [0], [low], ss, sc)[0]
+    worst = 0.0
+    for y in range(24):
+        for x in range(32):
+            px, py = (x - 0.5) * 0.5, (y - 0.5) * 0.5
+            qx0, qy0 = int(np.floor(px + 0.5)) if px >= 0 else -int(np.floor(-px + 0.5)), \
+                int(np.floor(py + 0.5)) if py >= 0 else -int(np.floor(-py + 0.5))
+            ip = hi_g[y, x] / 255.0
+            sw = sd = 0.0
+            for qy in range(qy0 - 2, qy0 + 3):
+                for qx in range(qx0 - 2, qx0 + 3):
+                    if 0 <= qx < 16 and 0 <= qy < 12:
+                        ds = (qx - px) ** 2 + (qy - py) ** 2
+                        wq = math.exp(-ds / (2 * ss * ss) - (ip - lo_g[qy, qx] / 255.0) ** 2 / (2 * sc * sc))
+                        sw += wq
+                        sd += wq * float(low.depth[qy, qx])
+            worst = max(worst, abs(up.depth[y, x] - sd / sw) / (sd / sw))
+    assert worst < 2e-6
+    d = rng.uniform(5.0, 9.0, (24, 32)).astype(np.float32)
+    st = abi.ViewState(32, 24)
+    st.depth[...] = d
+    st.normal[...] = np.array([0.0, 0.0, -1.0], np.float32)
+    m = o.median_filter(1, [0], [st])[0]
+    for y in range(24):
+        for x in range(32):
+            win = np.sort(d[max(0, y - 2):y + 3, max(0, x - 2):x + 3].reshape(-1))
+            assert m.depth[y, x] == win[(win.size - 1) // 2]
