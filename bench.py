@@ -69,25 +69,41 @@ def make_workload(width, height, nviews, noise, seed):
     src = [[j for j in range(nviews) if j != v] for v in range(nviews)]
     pmp, fl = abi.fast_profile()
     inp = scenes.kernel_input(sc, src, [(15.0, 120.0)] * nviews, pmp, fl)
-    return sc, inp
+    return sc, inp, src
 
 
-def cpu_baseline(inp, rows, threads, photometric_iters):
-    """The CPU oracle (C, OpenMP over rows) on a bounded sample: a band of `rows` rows of view 0,
+BATCH_SIZES = [(960, 540), (1280, 720), (1920, 1080)]
+
+
+def make_batched_workload(nviews, noise, seed):
+    """48 views of a formation flight (16 stations x 3 cameras), resolutions drawn uniformly from
+    960x540 / 1280x720 / 1920x1080, 2-8 sources each (uniform), seeded."""
+    rng = np.random.default_rng(seed)
+    cams = scenes.formation_rig(nviews // 3, 3)
+    sizes = [BATCH_SIZES[k] for k in rng.integers(0, 3, len(cams))]
+    sc = scenes.make_mixed_scene(cams, sizes, seed=seed, noise_sigma=noise, noise_seed=seed + 1)
+    src = scenes.nearest_sources(sc, [int(k) for k in rng.integers(2, 9, len(cams))])
+    pmp, fl = abi.fast_profile()
+    inp = scenes.kernel_input(sc, src, [(15.0, 120.0)] * len(cams), pmp, fl)
+    return sc, inp, src
+
+
+def cpu_baseline(inp, rows, threads, photometric_iters, view=0):
+    """The CPU oracle (C, OpenMP over rows) on a bounded sample: a band of `rows` rows of `view`,
     random-init photometric run (evaluation of the band and its 24-row reach, then the iterations
     on the band only). Returns the rate in pixel-iterations per second (eval counted as one)."""
     import oracle
 
     o = oracle.Oracle(inp, threads=threads)
-    h, w = inp.level_shape(0, 0)
+    h, w = inp.level_shape(view, 0)
     st = abi.ViewState(w, h)
     p = abi.run_params(0, False, True, photometric_iters, 1, False)
     r0 = h // 2 - rows // 2
     t = time.perf_counter()
-    o.run_rows(0, st, p, r0, r0 + rows)
+    o.run_rows(view, st, p, r0, r0 + rows)
     dt = time.perf_counter() - t
     px_iters = w * rows * photometric_iters + w * min(h, rows + 48)
-    return px_iters / dt, dt, f"view 0, rows {r0}..{r0 + rows} of {w}x{h}, random init + {photometric_iters} photometric iterations"
+    return px_iters / dt, dt, f"view {view}, rows {r0}..{r0 + rows} of {w}x{h}, random init + {photometric_iters} photometric iterations"
 
 
 def main():
@@ -95,12 +111,16 @@ def main():
     ap.add_argument("--gpus", type=int, default=1)
     ap.add_argument("--steps", type=int, default=3)
     ap.add_argument("--warmup", type=int, default=1)
+    ap.add_argument("--workload", choices=["zone7", "batched48"], default="zone7",
+                    help="zone7: 7 views 1920x1080, 6 sources each (default); batched48: 48 views of uneven "
+                         "resolution and source count in one batch, schedule captured in a hipGraph")
     ap.add_argument("--width", type=int, default=1920)
     ap.add_argument("--height", type=int, default=1080)
-    ap.add_argument("--views", type=int, default=7)
+    ap.add_argument("--views", type=int, default=None)
     ap.add_argument("--photometric-iters", type=int, default=7)
     ap.add_argument("--geometric-rounds", type=int, default=1)
     ap.add_argument("--geometric-iters", type=int, default=2)
+    ap.add_argument("--graph", type=int, default=None, help="1 = replay the schedule as one hipGraph (default: batched48 only)")
     ap.add_argument("--cpu-rows", type=int, default=320)
     ap.add_argument("--no-cpu", action="store_true")
     ap.add_argument("--calibrate-copy", type=int, default=0, metavar="MIB",
@@ -115,16 +135,27 @@ def main():
 
     if args.calibrate_copy:
         copy_calibration(args.calibrate_copy)
+    batched = args.workload == "batched48"
+    use_graph = args.graph if args.graph is not None else int(batched)
     t0 = time.perf_counter()
-    sc, inp = make_workload(args.width, args.height, args.views, 0.01, seed=1)
-    log(f"scene {args.views} x {args.width}x{args.height} rendered in {time.perf_counter() - t0:.1f} s")
-    views = list(range(args.views))
-    npix = args.width * args.height
+    if batched:
+        nviews = args.views or 48
+        sc, inp, src = make_batched_workload(nviews, 0.01, seed=3)
+    else:
+        nviews = args.views or 7
+        sc, inp, src = make_workload(args.width, args.height, nviews, 0.01, seed=1)
+    sizes = [inp.level_shape(v, 0) for v in range(nviews)]
+    npix_v = [h * w for (h, w) in sizes]
+    job_mpx = sum(npix_v) / 1e6
+    log(f"scene {nviews} views, {job_mpx:.1f} Mpx rendered in {time.perf_counter() - t0:.1f} s")
+    views = list(range(nviews))
     sched = abi.Schedule(photometric_iters=args.photometric_iters, geometric_rounds=args.geometric_rounds,
                          geometric_iters=args.geometric_iters)
-    be = pm.HipPatchMatch(0)
+    opts = abi.c3d_pm_options()
+    opts.use_graph = use_graph
+    be = pm.HipPatchMatch(0, opts)
     sess = be.begin(inp)
-    log(f"session open on {be.name()}")
+    log(f"session open on {be.name()}, graph {use_graph}")
     for i in range(args.warmup):
         sess.densify(views, sched, download=False)
         log(f"warmup {i + 1}/{args.warmup} done")
@@ -140,26 +171,37 @@ def main():
     device_sync()
     wall = time.perf_counter() - t_all
     ms_per_step = wall / args.steps * 1e3
-    job_mpx = args.views * npix / 1e6
     value = job_mpx / (ms_per_step / 1e3)
+    if use_graph:
+        # a graph replay has no per-launch events: one eager run outside the timed region gives the
+        # half-step launch times and the stage split
+        sess.close()
+        be.close()
+        be = pm.HipPatchMatch(0)
+        sess = be.begin(inp)
+        _, _, _, tm = sess.densify(views, sched, download=False)
+        tms = [tm]
+        log(f"eager run for kernel times: {tm.total_ms:.1f} ms")
 
     # dominant kernel: the red-black half step. Algorithmic work per pixel-iteration (one pixel
     # updated once by a red + black pair): S = 15 nsrc WN^2 bilinear samples (upper bound: 9
     # propagated + 6 refinement hypotheses over every source), 28 FLOP each; compulsory HBM bytes
     # 58 + 2 (1 + nsrc), plus 8 nsrc for geometric iterations. One launch updates npix/2 pixels
     # of every view.
-    nsrc = args.views - 1
     wn = (2 * inp.pm.window_radius) // inp.pm.window_step + 1
     n_photo = 2 * args.photometric_iters
     n_geo = 2 * args.geometric_iters * args.geometric_rounds
-    px_per_launch = args.views * npix / 2
-    bytes_photo = 58 + 2 * (1 + nsrc)
-    bytes_geo = bytes_photo + 8 * nsrc
-    avg_bytes = (n_photo * bytes_photo + n_geo * bytes_geo) / (n_photo + n_geo) * px_per_launch
-    flop = FLOP_PER_SAMPLE * 15 * nsrc * wn * wn * px_per_launch
+    bytes_photo = sum(p / 2 * (58 + 2 * (1 + len(s))) for p, s in zip(npix_v, src))
+    bytes_geo = sum(p / 2 * (58 + 2 * (1 + len(s)) + 8 * len(s)) for p, s in zip(npix_v, src))
+    avg_bytes = (n_photo * bytes_photo + n_geo * bytes_geo) / (n_photo + n_geo)
+    flop = sum(FLOP_PER_SAMPLE * 15 * len(s) * wn * wn * p / 2 for p, s in zip(npix_v, src))
     half_ms = sum(t.half_kernel_ms for t in tms) / sum(t.half_launches for t in tms)
     achieved_gbs = avg_bytes / (half_ms / 1e3) / 1e9
     achieved_tflops = flop / (half_ms / 1e3) / 1e12
+    nsrc_desc = f"{len(src[0])} src each" if len({len(s) for s in src}) == 1 else \
+        f"{min(len(s) for s in src)}-{max(len(s) for s in src)} src"
+    size_desc = f"{sizes[0][1]}x{sizes[0][0]}" if len(set(sizes)) == 1 else \
+        "mixed " + "/".join(f"{w}x{h}" for (w, h) in BATCH_SIZES)
     line = {
         "metric": "depth-map Mpx/s at <=1% rel. depth error vs plain PatchStereo; HBM GB/s",
         "value": round(value, 3),
@@ -174,16 +216,16 @@ def main():
         "dtype": "f32",
         "data": "synthetic (seeded fBm terrain, rendered on the host)",
         "config": {
-            "workload": f"plain PatchStereo, {args.views} views {args.width}x{args.height} u8, 1 ref + {nsrc} src each, "
+            "workload": f"plain PatchStereo, {nviews} views {size_desc} u8, 1 ref + {nsrc_desc}, "
                         f"window 5 step 2, {args.photometric_iters} photometric + {args.geometric_rounds}x{args.geometric_iters} "
-                        f"geometric iterations, read-out filter, depth 15-120 m",
-            "views": args.views,
-            "width": args.width,
-            "height": args.height,
-            "sources_per_view": nsrc,
+                        f"geometric iterations, read-out filter, depth 15-120 m" + (", hipGraph replay" if use_graph else ""),
+            "views": nviews,
+            "megapixels": round(job_mpx, 3),
+            "sources": [len(s) for s in src] if batched else len(src[0]),
             "photometric_iters": args.photometric_iters,
             "geometric_rounds": args.geometric_rounds,
             "geometric_iters": args.geometric_iters,
+            "graph": use_graph,
         },
         "roofline": {
             "bound": "hbm",
@@ -214,8 +256,9 @@ def main():
     sess.close()
     if not args.no_cpu:
         threads = min(os.cpu_count() or 1, 16)
-        rate, dt, sample = cpu_baseline(inp, args.cpu_rows, threads, args.photometric_iters)
-        job_px_iters = args.views * npix * (args.photometric_iters + 1 + args.geometric_rounds * (args.geometric_iters + 1) + 1)
+        view = max(range(nviews), key=lambda v: npix_v[v])
+        rate, dt, sample = cpu_baseline(inp, args.cpu_rows, threads, args.photometric_iters, view)
+        job_px_iters = sum(npix_v) * (args.photometric_iters + 1 + args.geometric_rounds * (args.geometric_iters + 1) + 1)
         cpu_mpxs = job_mpx / (job_px_iters / rate)
         line["cpu_baseline"] = {
             "value": round(cpu_mpxs, 5),

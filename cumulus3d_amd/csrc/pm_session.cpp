@@ -165,6 +165,15 @@ struct c3d_pm_session {
     std::vector<hipEvent_t> marks;
     std::vector<int> mark_kind;
     size_t marks_used = 0;
+    // c3d_pm_densify: per-scale batch tables and snapshot offsets, uploaded before the device work
+    DevBuf<BatchItem> d_batch_lv;
+    DevBuf<long long> d_snap_lv;
+    std::vector<BatchItem> h_batch_lv;
+    std::vector<long long> h_snap_lv;
+    std::vector<uint8_t> h_need;
+    // the captured schedule (options.use_graph) and the key it was captured for
+    hipGraphExec_t gexec = nullptr;
+    std::vector<uint8_t> gkey;
 };
 
 namespace {
@@ -622,6 +631,9 @@ void c3d_pm_end(c3d_pm_session* s) {
         if (e) (void)hipEventDestroy(e);
     for (auto& e : s->pool) (void)hipEventDestroy(e);
     for (auto& e : s->marks) (void)hipEventDestroy(e);
+    if (s->gexec) (void)hipGraphExecDestroy(s->gexec);
+    s->d_batch_lv.release();
+    s->d_snap_lv.release();
     delete s;
 }
 
@@ -818,6 +830,180 @@ int c3d_pm_median(c3d_pm_session* s, uint32_t level, const uint32_t* views, uint
     return C3D_OK;
 }
 
+}  // extern "C"
+
+namespace {
+
+// Everything the device-resident schedule needs, laid out before any device work is queued.
+struct DensifyPlan {
+    uint32_t top = 0, n = 0;
+    std::vector<long long> npix;  // per scale
+    long long npix_max = 0;
+    int max_nsrc = 1;
+    bool any_prior = false;
+    uint32_t half_launches = 0;
+};
+
+// Per-scale batch tables (views back to back, low_off = the view's offset one scale down) and
+// snapshot offset tables, written to the device in one upload.
+int plan_densify(c3d_pm_session* s, const uint32_t* views, uint32_t n, const c3d_schedule* sch, DensifyPlan& pl) {
+    pl.top = (uint32_t)s->nl - 1;
+    pl.n = n;
+    pl.npix.assign(pl.top + 1, 0);
+    s->h_batch_lv.assign((size_t)(pl.top + 1) * n, BatchItem{});
+    s->h_snap_lv.assign((size_t)(pl.top + 1) * s->nv, -1);
+    for (uint32_t l = 0; l <= pl.top; ++l) {
+        long long off = 0;
+        for (uint32_t i = 0; i < n; ++i) {
+            const LevelDesc& L = s->lv[views[i] * s->nl + l];
+            BatchItem& it = s->h_batch_lv[(size_t)l * n + i];
+            it.off = off;
+            it.view = (int)views[i];
+            it.w = L.w;
+            it.h = L.h;
+            it.low_off = l > 0 ? (int)s->h_batch_lv[(size_t)(l - 1) * n + i].off : 0;
+            s->h_snap_lv[(size_t)l * s->nv + views[i]] = off;
+            off += (long long)L.w * L.h;
+            pl.max_nsrc = std::max(pl.max_nsrc, s->vd[views[i]].nsrc);
+        }
+        pl.npix[l] = off;
+        pl.npix_max = std::max(pl.npix_max, off);
+    }
+    pl.any_prior = s->nl > 1 && s->pm_copy.weak_texture_var > 0.f;
+    pl.half_launches = 2 * (sch->photometric_iters + pl.top * sch->photometric_iters_upper +
+                            (pl.top + 1) * sch->geometric_rounds * sch->geometric_iters);
+    s->h_need.resize(n);
+    for (uint32_t i = 0; i < n; ++i) {
+        uint32_t nd = std::min<uint32_t>(s->filter_copy.min_num_consistent, (uint32_t)s->vd[views[i]].nsrc);
+        s->h_need[i] = (uint8_t)std::min<uint32_t>(std::max<uint32_t>(nd, 1u), 255u);
+    }
+    const long long m = pl.npix_max;
+    HIP_TRY(s->plane.reserve(m));
+    HIP_TRY(s->cost.reserve(m));
+    HIP_TRY(s->bview.reserve(m));
+    HIP_TRY(s->count.reserve(m));
+    HIP_TRY(s->snap.reserve(m));
+    HIP_TRY(s->plane2.reserve(m));
+    HIP_TRY(s->need.reserve(n));
+    if (s->nl > 1) {
+        HIP_TRY(s->low.reserve(m));
+        HIP_TRY(s->up.reserve(m));
+        HIP_TRY(s->cinit.reserve(m));
+    }
+    if (pl.any_prior || s->filter_copy.median_filter) HIP_TRY(s->prior.reserve(m));
+    HIP_TRY(s->d_batch_lv.reserve(s->h_batch_lv.size()));
+    HIP_TRY(s->d_snap_lv.reserve(s->h_snap_lv.size()));
+    hipStream_t st = s->be->stream;
+    HIP_TRY(hipMemcpyAsync(s->d_batch_lv.p, s->h_batch_lv.data(), s->h_batch_lv.size() * sizeof(BatchItem), hipMemcpyHostToDevice, st));
+    HIP_TRY(hipMemcpyAsync(s->d_snap_lv.p, s->h_snap_lv.data(), s->h_snap_lv.size() * sizeof(long long), hipMemcpyHostToDevice, st));
+    HIP_TRY(hipMemcpyAsync(s->need.p, s->h_need.data(), n, hipMemcpyHostToDevice, st));
+    HIP_TRY(hipStreamSynchronize(st));
+    return C3D_OK;
+}
+
+// Kernel parameters of scale `level` over the planned batch.
+KParams plan_params(c3d_pm_session* s, const DensifyPlan& pl, uint32_t level, bool geometric) {
+    std::vector<BatchItem> none;
+    KParams k = params_for(s, level, geometric, none, pl.max_nsrc);
+    k.nitems = (int)pl.n;
+    k.batch = s->d_batch_lv.p + (size_t)level * pl.n;
+    k.snap_off = s->d_snap_lv.p + (size_t)level * s->nv;
+    return k;
+}
+
+// Queues the whole schedule (densify.rs:350-454) on the session stream: kernels and
+// device-to-device copies only, so that it can be captured into a graph. With `timed`, stage
+// boundaries and every half-step launch are bracketed by events.
+int enqueue_densify(c3d_pm_session* s, const DensifyPlan& pl, const c3d_schedule* sch, bool timed) {
+    hipStream_t st = s->be->stream;
+    const BatchItem* items_all = s->h_batch_lv.data();
+    uint32_t run_id = sch->first_run_id;
+    for (uint32_t level = 0; level <= pl.top; ++level) {
+        // params.rs:476-506 through the schedule struct; use_prior densify.rs:366
+        const bool use_prior = level > 0 && pl.any_prior;
+        const BatchItem* items = items_all + (size_t)level * pl.n;
+        const std::vector<BatchItem> iv(items, items + pl.n);
+        const long long npix = pl.npix[level];
+        if (timed) MARK(0);
+        if (level == 0) {
+            KParams k = plan_params(s, pl, 0, false);
+            k.run_id = (int)run_id++;
+            k.random_init = 1;
+            int rc = launch_run(s, k, iv, s->pm_copy, false, sch->photometric_iters, timed);
+            if (rc) return rc;
+        } else {
+            // joint bilateral upsampling of the lower scale (patchmatch.cu:741-800)
+            HIP_TRY(hipMemcpyAsync(s->low.p, s->plane.p, (size_t)pl.npix[level - 1] * sizeof(float4), hipMemcpyDeviceToDevice, st));
+            KParams u = plan_params(s, pl, level, false);
+            u.prior = s->low.p;
+            u.up_inv2s = 1.0f / (2.0f * sch->jbu_sigma_spatial * sch->jbu_sigma_spatial);
+            u.up_inv2c = 1.0f / (2.0f * sch->jbu_sigma_color * sch->jbu_sigma_color);
+            u.ntiles = pix_tiles(items, (int)pl.n);
+            HIP_TRY(launch_upsample(u, st));
+            if (use_prior)
+                HIP_TRY(hipMemcpyAsync(s->prior.p, s->plane.p, (size_t)npix * sizeof(float4), hipMemcpyDeviceToDevice, st));
+            HIP_TRY(hipMemcpyAsync(s->up.p, s->plane.p, (size_t)npix * sizeof(float4), hipMemcpyDeviceToDevice, st));
+            // detail restorer (densify.rs:373-409): c_init, photometric run, c_photo, merge
+            KParams e = plan_params(s, pl, level, false);
+            e.ntiles = pix_tiles(items, (int)pl.n);
+            HIP_TRY(s->ks->eval(e, 0, st));
+            HIP_TRY(hipMemcpyAsync(s->cinit.p, s->cost.p, (size_t)npix * sizeof(float), hipMemcpyDeviceToDevice, st));
+            KParams k = plan_params(s, pl, level, false);
+            k.run_id = (int)run_id++;
+            k.random_init = sch->restorer_random_init ? 1 : 0;
+            k.use_prior = use_prior ? 1 : 0;
+            int rc = launch_run(s, k, iv, s->pm_copy, false, sch->photometric_iters_upper, timed);
+            if (rc) return rc;
+            HIP_TRY(s->ks->eval(e, 0, st));
+            HIP_TRY(launch_restore(s->plane.p, s->up.p, s->cinit.p, s->cost.p, sch->restorer_threshold, npix, st));
+        }
+        if (timed) MARK(1);
+        for (uint32_t r = 0; r < sch->geometric_rounds; ++r) {
+            HIP_TRY(launch_extract_depth(s->plane.p, s->snap.p, npix, st));
+            KParams g = plan_params(s, pl, level, true);
+            g.run_id = (int)run_id++;
+            g.random_init = 0;
+            g.use_prior = use_prior ? 1 : 0;
+            int rc = launch_run(s, g, iv, s->pm_copy, true, sch->geometric_iters, timed);
+            if (rc) return rc;
+        }
+    }
+    // read-out at the top scale (densify.rs:428-454)
+    if (timed) MARK(2);
+    const BatchItem* items = items_all + (size_t)pl.top * pl.n;
+    const long long npix = pl.npix[pl.top];
+    if (s->filter_copy.median_filter) {
+        HIP_TRY(hipMemcpyAsync(s->prior.p, s->plane.p, (size_t)npix * sizeof(float4), hipMemcpyDeviceToDevice, st));
+        KParams m = plan_params(s, pl, pl.top, false);
+        m.ntiles = pix_tiles(items, (int)pl.n);
+        HIP_TRY(launch_median(m, st));
+    }
+    HIP_TRY(launch_extract_depth(s->plane.p, s->snap.p, npix, st));
+    KParams f = plan_params(s, pl, pl.top, true);
+    f.ntiles = pix_tiles(items, (int)pl.n);
+    HIP_TRY(s->ks->filter(f, st));
+    HIP_TRY(launch_threshold(f, s->need.p, s->plane2.p, st));
+    if (timed) MARK(-1);
+    return C3D_OK;
+}
+
+// Identity of a captured schedule: the views, the schedule and every device address it reads.
+std::vector<uint8_t> graph_key(c3d_pm_session* s, const uint32_t* views, uint32_t n, const c3d_schedule* sch) {
+    std::vector<uint8_t> k;
+    auto put = [&](const void* p, size_t b) { k.insert(k.end(), (const uint8_t*)p, (const uint8_t*)p + b); };
+    put(&n, sizeof n);
+    put(views, n * sizeof(uint32_t));
+    put(sch, sizeof *sch);
+    const void* ptrs[] = {s->plane.p, s->cost.p, s->bview.p, s->count.p, s->snap.p, s->plane2.p, s->need.p, s->low.p,
+                          s->up.p, s->cinit.p, s->prior.p, s->d_batch_lv.p, s->d_snap_lv.p};
+    put(ptrs, sizeof ptrs);
+    return k;
+}
+
+}  // namespace
+
+extern "C" {
+
 int c3d_pm_densify(c3d_pm_session* s, const uint32_t* views, uint32_t n, const c3d_schedule* sch, c3d_view_state* out,
                    uint8_t* const* counts, float* const* raw_depth, c3d_timing* timing) {
     if (!s || !sch || !views || n == 0) return fail(C3D_INVALID_ARGUMENT, "null argument");
@@ -832,162 +1018,92 @@ int c3d_pm_densify(c3d_pm_session* s, const uint32_t* views, uint32_t n, const c
     if (s->nl > 1 && !(sch->jbu_sigma_spatial > 0.f && sch->jbu_sigma_color > 0.f))
         return fail(C3D_INVALID_ARGUMENT, "joint bilateral upsampling needs positive sigmas");
     hipStream_t st = s->be->stream;
-    // every buffer at the largest scale, allocated up front (no reallocation between scales)
-    long long npix_max = 0;
-    for (uint32_t l = 0; l <= top; ++l) {
-        long long p = 0;
-        for (uint32_t i = 0; i < n; ++i) p += (long long)s->lv[views[i] * s->nl + l].w * s->lv[views[i] * s->nl + l].h;
-        npix_max = std::max(npix_max, p);
-    }
-    HIP_TRY(s->plane.reserve(npix_max));
-    HIP_TRY(s->cost.reserve(npix_max));
-    HIP_TRY(s->bview.reserve(npix_max));
-    HIP_TRY(s->count.reserve(npix_max));
-    HIP_TRY(s->snap.reserve(npix_max));
-    HIP_TRY(s->plane2.reserve(npix_max));
-    HIP_TRY(s->need.reserve(n));
-    if (s->nl > 1) {
-        HIP_TRY(s->low.reserve(npix_max));
-        HIP_TRY(s->up.reserve(npix_max));
-        HIP_TRY(s->cinit.reserve(npix_max));
-    }
-    const bool any_prior = s->nl > 1 && s->pm_copy.weak_texture_var > 0.f;
-    if (any_prior) HIP_TRY(s->prior.reserve(npix_max));
+    DensifyPlan pl;
+    rc = plan_densify(s, views, n, sch, pl);
+    if (rc) return rc;
     s->snap_valid = false;  // the snapshot buffer now holds device-made maps
-    std::vector<uint8_t> need(n);
-    for (uint32_t i = 0; i < n; ++i) {
-        uint32_t nd = std::min<uint32_t>(s->filter_copy.min_num_consistent, (uint32_t)s->vd[views[i]].nsrc);
-        need[i] = (uint8_t)std::min<uint32_t>(std::max<uint32_t>(nd, 1u), 255u);
-    }
-    HIP_TRY(hipMemcpyAsync(s->need.p, need.data(), n, hipMemcpyHostToDevice, st));
-    const bool time_halves = timing != nullptr;
     s->pool_used = 0;
     s->marks_used = 0;
-    std::vector<std::vector<long long>> snap_offs(top + 1, std::vector<long long>((size_t)s->nv, -1));
-    std::vector<BatchItem> items, prev;
-    long long npix = 0, prev_npix = 0;
-    int max_nsrc = 1;
-    uint32_t run_id = sch->first_run_id;
-    for (uint32_t level = 0; level <= top; ++level) {
-        // params.rs:476-506 through the schedule struct; use_prior densify.rs:366
-        const bool use_prior = level > 0 && any_prior;
-        MARK(0);
-        if (level > 0) {
-            // the lower scale's layout and planes move aside before the new batch layout is written
-            prev = items;
-            prev_npix = npix;
-            HIP_TRY(hipMemcpyAsync(s->low.p, s->plane.p, (size_t)prev_npix * sizeof(float4), hipMemcpyDeviceToDevice, st));
+    if (s->be->opts.use_graph) {
+        // the whole schedule as one graph, captured once per (views, schedule, buffers) and replayed
+        std::vector<uint8_t> key = graph_key(s, views, n, sch);
+        if (!s->gexec || key != s->gkey) {
+            if (s->gexec) HIP_TRY(hipGraphExecDestroy(s->gexec));
+            s->gexec = nullptr;
+            HIP_TRY(hipStreamBeginCapture(st, hipStreamCaptureModeThreadLocal));
+            rc = enqueue_densify(s, pl, sch, false);
+            hipGraph_t g = nullptr;
+            hipError_t e = hipStreamEndCapture(st, &g);
+            if (rc) {
+                if (g) (void)hipGraphDestroy(g);
+                return rc;
+            }
+            if (e != hipSuccess) return fail(C3D_DEVICE, std::string("HIP: graph capture: ") + hipGetErrorString(e));
+            e = hipGraphInstantiate(&s->gexec, g, nullptr, nullptr, 0);
+            (void)hipGraphDestroy(g);
+            if (e != hipSuccess) {
+                s->gexec = nullptr;
+                return fail(C3D_DEVICE, std::string("HIP: graph instantiate: ") + hipGetErrorString(e));
+            }
+            s->gkey = key;
         }
-        rc = set_batch(s, views, 0, n, level, items, npix, max_nsrc);
+        HIP_TRY(hipEventRecord(s->ev[0], st));
+        HIP_TRY(hipGraphLaunch(s->gexec, st));
+        HIP_TRY(hipEventRecord(s->ev[1], st));
+        HIP_TRY(hipEventSynchronize(s->ev[1]));
+        if (timing) {
+            float tot = 0.f;
+            HIP_TRY(hipEventElapsedTime(&tot, s->ev[0], s->ev[1]));
+            *timing = c3d_timing{};
+            timing->total_ms = tot;
+            timing->half_launches = pl.half_launches;
+        }
+    } else {
+        rc = enqueue_densify(s, pl, sch, timing != nullptr);
         if (rc) return rc;
-        if (level == 0) {
-            KParams k = params_for(s, 0, false, items, max_nsrc);
-            k.run_id = (int)run_id++;
-            k.random_init = 1;
-            rc = launch_run(s, k, items, s->pm_copy, false, sch->photometric_iters, time_halves);
-            if (rc) return rc;
-        } else {
-            // joint bilateral upsampling of the lower scale (patchmatch.cu:741-800)
-            for (size_t i = 0; i < items.size(); ++i) items[i].low_off = (int)prev[i].off;
-            HIP_TRY(hipMemcpyAsync(s->d_batch.p, items.data(), items.size() * sizeof(BatchItem), hipMemcpyHostToDevice, st));
-            KParams u = params_for(s, level, false, items, max_nsrc);
-            u.prior = s->low.p;
-            u.up_inv2s = 1.0f / (2.0f * sch->jbu_sigma_spatial * sch->jbu_sigma_spatial);
-            u.up_inv2c = 1.0f / (2.0f * sch->jbu_sigma_color * sch->jbu_sigma_color);
-            u.ntiles = pix_tiles(items.data(), (int)items.size());
-            HIP_TRY(launch_upsample(u, st));
-            if (use_prior)
-                HIP_TRY(hipMemcpyAsync(s->prior.p, s->plane.p, (size_t)npix * sizeof(float4), hipMemcpyDeviceToDevice, st));
-            HIP_TRY(hipMemcpyAsync(s->up.p, s->plane.p, (size_t)npix * sizeof(float4), hipMemcpyDeviceToDevice, st));
-            // detail restorer (densify.rs:373-409): c_init, photometric run, c_photo, merge
-            KParams e = params_for(s, level, false, items, max_nsrc);
-            e.ntiles = pix_tiles(items.data(), (int)items.size());
-            HIP_TRY(s->ks->eval(e, 0, st));
-            HIP_TRY(hipMemcpyAsync(s->cinit.p, s->cost.p, (size_t)npix * sizeof(float), hipMemcpyDeviceToDevice, st));
-            KParams k = params_for(s, level, false, items, max_nsrc);
-            k.run_id = (int)run_id++;
-            k.random_init = sch->restorer_random_init ? 1 : 0;
-            k.use_prior = use_prior ? 1 : 0;
-            rc = launch_run(s, k, items, s->pm_copy, false, sch->photometric_iters_upper, time_halves);
-            if (rc) return rc;
-            HIP_TRY(s->ks->eval(e, 0, st));
-            HIP_TRY(launch_restore(s->plane.p, s->up.p, s->cinit.p, s->cost.p, sch->restorer_threshold, npix, st));
-        }
-        MARK(1);
-        if (sch->geometric_rounds > 0) {
-            std::vector<long long>& offs = snap_offs[level];
-            for (auto& it : items) offs[it.view] = it.off;
-            HIP_TRY(hipMemcpyAsync(s->snap_off.p, offs.data(), offs.size() * sizeof(long long), hipMemcpyHostToDevice, st));
-        }
-        for (uint32_t r = 0; r < sch->geometric_rounds; ++r) {
-            HIP_TRY(launch_extract_depth(s->plane.p, s->snap.p, npix, st));
-            KParams g = params_for(s, level, true, items, max_nsrc);
-            g.run_id = (int)run_id++;
-            g.random_init = 0;
-            g.use_prior = use_prior ? 1 : 0;
-            rc = launch_run(s, g, items, s->pm_copy, true, sch->geometric_iters, time_halves);
+        HIP_TRY(hipStreamSynchronize(st));
+        if (timing) {
+            double acc[3] = {0.0, 0.0, 0.0};
+            for (size_t i = 0; i + 1 < s->marks_used; ++i) {
+                float ms = 0.f;
+                HIP_TRY(hipEventElapsedTime(&ms, s->marks[i], s->marks[i + 1]));
+                if (s->mark_kind[i] >= 0) acc[s->mark_kind[i]] += ms;
+            }
+            float tot = 0.f;
+            HIP_TRY(hipEventElapsedTime(&tot, s->marks[0], s->marks[s->marks_used - 1]));
+            timing->photometric_ms = acc[0];
+            timing->geometric_ms = acc[1];
+            timing->filter_ms = acc[2];
+            timing->total_ms = tot;
+            rc = pool_sum(s, &timing->half_kernel_ms, &timing->half_launches);
             if (rc) return rc;
         }
     }
-    // read-out at the top scale (densify.rs:428-454)
-    MARK(2);
-    if (s->filter_copy.median_filter) {
-        HIP_TRY(s->prior.reserve(npix_max));
-        HIP_TRY(hipMemcpyAsync(s->prior.p, s->plane.p, (size_t)npix * sizeof(float4), hipMemcpyDeviceToDevice, st));
-        KParams m = params_for(s, top, false, items, max_nsrc);
-        m.ntiles = pix_tiles(items.data(), (int)items.size());
-        HIP_TRY(launch_median(m, st));
-    }
-    {
-        std::vector<long long>& offs = snap_offs[top];
-        for (auto& it : items) offs[it.view] = it.off;
-        HIP_TRY(hipMemcpyAsync(s->snap_off.p, offs.data(), offs.size() * sizeof(long long), hipMemcpyHostToDevice, st));
-    }
-    HIP_TRY(launch_extract_depth(s->plane.p, s->snap.p, npix, st));
-    KParams f = params_for(s, top, true, items, max_nsrc);
-    f.ntiles = pix_tiles(items.data(), (int)items.size());
-    HIP_TRY(s->ks->filter(f, st));
-    HIP_TRY(launch_threshold(f, s->need.p, s->plane2.p, st));
-    MARK(-1);
-    HIP_TRY(hipEventSynchronize(s->marks[s->marks_used - 1]));
-    if (timing) {
-        double acc[3] = {0.0, 0.0, 0.0};
-        for (size_t i = 0; i + 1 < s->marks_used; ++i) {
-            float ms = 0.f;
-            HIP_TRY(hipEventElapsedTime(&ms, s->marks[i], s->marks[i + 1]));
-            if (s->mark_kind[i] >= 0) acc[s->mark_kind[i]] += ms;
-        }
-        float tot = 0.f;
-        HIP_TRY(hipEventElapsedTime(&tot, s->marks[0], s->marks[s->marks_used - 1]));
-        timing->photometric_ms = acc[0];
-        timing->geometric_ms = acc[1];
-        timing->filter_ms = acc[2];
-        timing->total_ms = tot;
-        rc = pool_sum(s, &timing->half_kernel_ms, &timing->half_launches);
-        if (rc) return rc;
-    }
+    const BatchItem* items = s->h_batch_lv.data() + (size_t)top * n;
+    const long long npix = pl.npix[top];
     if (out) {
         s->host4.resize((size_t)npix);
         s->host1.resize((size_t)npix);
         HIP_TRY(hipMemcpyAsync(s->host4.data(), s->plane2.p, (size_t)npix * sizeof(float4), hipMemcpyDeviceToHost, st));
         HIP_TRY(hipMemcpyAsync(s->host1.data(), s->cost.p, (size_t)npix * sizeof(float), hipMemcpyDeviceToHost, st));
         HIP_TRY(hipStreamSynchronize(st));
-        unpack(s->host4, items, out);
-        for (size_t i = 0; i < items.size(); ++i)
+        std::vector<BatchItem> iv(items, items + n);
+        unpack(s->host4, iv, out);
+        for (uint32_t i = 0; i < n; ++i)
             if (out[i].cost) std::memcpy(out[i].cost, s->host1.data() + items[i].off, (size_t)items[i].w * items[i].h * sizeof(float));
     }
     if (raw_depth) {
         std::vector<float> hd((size_t)npix);
         HIP_TRY(hipMemcpyAsync(hd.data(), s->snap.p, (size_t)npix * sizeof(float), hipMemcpyDeviceToHost, st));
         HIP_TRY(hipStreamSynchronize(st));
-        for (size_t i = 0; i < items.size(); ++i)
+        for (uint32_t i = 0; i < n; ++i)
             if (raw_depth[i]) std::memcpy(raw_depth[i], hd.data() + items[i].off, (size_t)items[i].w * items[i].h * sizeof(float));
     }
     if (counts) {
         std::vector<uint8_t> hc((size_t)npix);
         HIP_TRY(hipMemcpyAsync(hc.data(), s->count.p, (size_t)npix, hipMemcpyDeviceToHost, st));
         HIP_TRY(hipStreamSynchronize(st));
-        for (size_t i = 0; i < items.size(); ++i)
+        for (uint32_t i = 0; i < n; ++i)
             if (counts[i]) std::memcpy(counts[i], hc.data() + items[i].off, (size_t)items[i].w * items[i].h);
     }
     return C3D_OK;

@@ -539,3 +539,67 @@ def kernel_input(
         )
     pm = abi.PmParams(**{**pm.__dict__, "sigma_spatial": pm.sigma_s()})
     return abi.KernelInput(vd, pm, filt, seed)
+
+
+# ---------------------------------------------------------------- formation flights (batched workloads)
+
+
+def formation_rig(positions: int, cams_per_position: int = 3, spacing: float = 8.0, altitude: float = 30.0,
+                  ground_dist: float = 30.0, yaw_spread_deg: float = 30.0, cols: Optional[int] = None):
+    """A formation flight over the terrain: `positions` stations on a serpentine grid (`cols` per
+    row, `spacing` m apart) at `altitude` m, each carrying `cams_per_position` cameras that look
+    45 deg down (ground_dist = altitude), fanned by +-yaw_spread_deg about the flight heading.
+    Returns (center, target) pairs, station-major."""
+    cols = cols or max(1, int(round(math.sqrt(positions))))
+    rows = (positions + cols - 1) // cols
+    cams = []
+    for p in range(positions):
+        r, c = divmod(p, cols)
+        if r % 2:
+            c = cols - 1 - c
+        x = (c - (cols - 1) / 2.0) * spacing
+        y = (r - (rows - 1) / 2.0) * spacing - ground_dist
+        center = np.array([x, y, altitude])
+        for k in range(cams_per_position):
+            off = 0.0 if cams_per_position == 1 else (k / (cams_per_position - 1) - 0.5) * 2.0 * yaw_spread_deg
+            yaw = math.radians(off)
+            fwd = np.array([math.sin(yaw), math.cos(yaw), 0.0])
+            cams.append((center, center + fwd * ground_dist - np.array([0.0, 0.0, altitude])))
+    return cams
+
+
+def make_mixed_scene(cameras: Sequence[tuple], sizes: Sequence[tuple], focal_1080: float = 500.0, seed: int = 1,
+                     noise_sigma: float = 0.0, noise_seed: int = 2) -> DenseScene:
+    """Terrain views of uneven resolution: camera i is rendered at sizes[i] = (w, h) with focal
+    focal_1080 * w / 1920 (the same field of view at every resolution)."""
+    views = [None] * len(cameras)
+    gd, gn = [None] * len(cameras), [None] * len(cameras)
+    for size in sorted(set(tuple(s) for s in sizes)):
+        idx = [i for i, s in enumerate(sizes) if tuple(s) == size]
+        w, h = size
+        sub = make_terrain_scene([cameras[i] for i in idx], w, h, focal=focal_1080 * w / 1920.0, seed=seed,
+                                 noise_sigma=noise_sigma, noise_seed=noise_seed,
+                                 names=[f"terrain_{seed}_{i}.png" for i in idx])
+        for j, i in enumerate(idx):
+            views[i], gd[i], gn[i] = sub.views[j], sub.gt_depth[j], sub.gt_normal[j]
+    return DenseScene(views, [], gd, gn)
+
+
+def nearest_sources(scene: DenseScene, counts: Sequence[int], min_baseline: float = 0.5) -> List[List[int]]:
+    """Per view, the `counts[v]` other views whose ground footprint centres (the viewing ray's hit
+    on the z = 0 plane) are closest, ties by index, among the views at least `min_baseline` m away
+    (cameras of one station share a centre and cannot triangulate): overlapping views of a
+    formation flight."""
+    foot = []
+    for v in scene.views:
+        c = v.center()
+        d = v.r[2]  # viewing direction in world coordinates
+        t = -c[2] / d[2] if abs(d[2]) > 1e-9 else 0.0
+        foot.append(c + t * d)
+    out = []
+    for i in range(len(scene.views)):
+        ci = scene.views[i].center()
+        order = sorted((float(np.linalg.norm(foot[i] - foot[j])), j) for j in range(len(scene.views))
+                       if j != i and np.linalg.norm(scene.views[j].center() - ci) >= min_baseline)
+        out.append([j for _, j in order[: counts[i]]])
+    return out
