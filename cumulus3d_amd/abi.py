@@ -167,6 +167,7 @@ class c3d_timing(C.Structure):
         ("half_kernel_ms", C.c_double),
         ("half_launches", C.c_uint32),
         ("pad0", C.c_uint32),
+        ("init_ms", C.c_double),
     ]
 
 

@@ -1,6 +1,7 @@
 """Builds the native libraries of cumulus3d_amd in-tree (cumulus3d_amd/lib/):
 
-* libc3d_pm.so        the gfx950 PatchStereo engine (HIP kernels + C ABI of include/c3d_pm.h)
+* libc3d_pm.so        the gfx950 PatchStereo engine and its learned initialiser (HIP kernels + C ABI of
+                      include/c3d_pm.h and include/c3d_init.h)
 * libc3d_scenes.so    the synthetic terrain renderer used by tests and benchmarks (host C)
 
 All HIP code is compiled for gfx950 only, with -ffp-contract=off (every multiply-add is written out)
@@ -42,11 +43,11 @@ def _engine(name: str, defines: list[str]) -> str:
     os.makedirs(os.path.join(LIB, "obj"), exist_ok=True)
     objs = []
     jobs = []
-    for src in ("pm_kernels.hip", "pm_session.cpp"):
+    for src in ("pm_kernels.hip", "pm_session.cpp", "nn_kernels.hip", "nn_net.cpp"):
         obj = os.path.join(LIB, "obj", f"{name}_{os.path.splitext(src)[0]}.o")
         jobs.append([HIPCC, *HIP_FLAGS, *defines, "-c", os.path.join(CSRC, src), "-o", obj])
         objs.append(obj)
-    with ThreadPoolExecutor(2) as ex:
+    with ThreadPoolExecutor(4) as ex:
         list(ex.map(_run, jobs))
     out = os.path.join(LIB, f"lib{name}.so")
     _run([HIPCC, "--offload-arch=gfx950", "-shared", "-fPIC", *objs, "-o", out + ".tmp"])

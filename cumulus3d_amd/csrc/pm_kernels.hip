@@ -331,7 +331,8 @@ __global__ __launch_bounds__(256) void k_eval(KParams P, int mode) {
     const long long base = it.off;
     f3 ray = pixel_ray(R, x, y);
     float4 h = P.plane[base + pix];
-    if (mode == 1 && P.random_init) {
+    // random_init 1: every pixel (patchmatch.cu:668-676); 2: pixels without a positive depth
+    if (mode == 1 && (P.random_init == 1 || (P.random_init == 2 && !(h.w > 0.f)))) {
         unsigned long long st = rng_state(P.seed, vd.key, P.level, P.run_id, 0xFFFF, 0, (unsigned long long)pix);
         float d = random_depth(st, vd.dmin, vd.dmax);
         f3 n = random_normal(st, ray);

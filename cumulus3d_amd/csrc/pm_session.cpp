@@ -23,7 +23,11 @@
 #include <string>
 #include <vector>
 
+#include "../../include/c3d_init.h"
 #include "../../include/c3d_pm.h"
+#include "backend.hpp"
+#include "nn_kernels.hpp"
+#include "nn_net.hpp"
 #include "pm_kernels.hpp"
 
 using namespace c3d;
@@ -112,13 +116,9 @@ struct DevBuf {
 
 }  // namespace
 
-struct c3d_pm_backend {
-    int device = 0;
-    std::string name;
-    c3d_pm_options opts{};
-    std::mutex lock;
-    hipStream_t stream = nullptr;
-};
+namespace c3d {
+int set_error(int code, const std::string& msg) { return fail(code, msg); }
+}  // namespace c3d
 
 struct c3d_pm_session {
     c3d_pm_backend* be = nullptr;
@@ -127,6 +127,8 @@ struct c3d_pm_session {
     int nv = 0, nl = 0;
     std::vector<LevelDesc> lv;
     std::vector<ViewDesc> vd;
+    std::vector<int> h_pair_src;        // [view][MAXSRC] source view index
+    std::vector<PairRec> h_pair_rt;     // [view][MAXSRC] reference -> source geometry
     // device tables
     DevBuf<uint8_t> gray;
     DevBuf<uint32_t> quad;
@@ -555,6 +557,9 @@ int c3d_pm_begin(c3d_pm_backend* be, const c3d_kernel_input* in, c3d_pm_session*
         d.dmax = kv.depth_max;
         d.nsrc = (int)kv.num_sources;
     }
+    s->h_pair_rt = prs;
+    s->h_pair_src.assign((size_t)nv * MAXSRC, 0);
+    for (size_t i = 0; i < prs.size(); ++i) s->h_pair_src[i] = prs[i].src;
     if (s->d_lv.reserve(s->lv.size()) != hipSuccess || s->d_pairs.reserve(prs.size()) != hipSuccess ||
         s->d_lpairs.reserve(lps.size()) != hipSuccess || s->d_views.reserve(s->vd.size()) != hipSuccess ||
         s->snap_off.reserve((size_t)nv) != hipSuccess || s->snap.reserve(1) != hipSuccess)
@@ -658,7 +663,7 @@ int c3d_pm_run(c3d_pm_session* s, const uint32_t* views, uint32_t n, c3d_view_st
         bool use_prior = p->use_prior != 0;
         for (uint32_t i = a; i < b; ++i)
             if (!states[i].prior) use_prior = false;
-        if (!p->random_init) {
+        if (p->random_init != 1) {
             pack(s, states + a, items, npix, false);
             HIP_TRY(hipMemcpyAsync(s->plane.p, s->host4.data(), (size_t)npix * sizeof(float4), hipMemcpyHostToDevice, st));
             HIP_TRY(hipStreamSynchronize(st));
@@ -671,7 +676,7 @@ int c3d_pm_run(c3d_pm_session* s, const uint32_t* views, uint32_t n, c3d_view_st
         }
         KParams k = params_for(s, p->level, p->geometric != 0, items, max_nsrc);
         k.run_id = (int)p->run_id;
-        k.random_init = p->random_init ? 1 : 0;
+        k.random_init = p->random_init == 1 ? 1 : (p->random_init == 2 ? 2 : 0);
         k.use_prior = use_prior ? 1 : 0;
         rc = launch_run(s, k, items, s->pm_copy, p->geometric != 0, p->iterations, false);
         if (rc) return rc;
@@ -1089,6 +1094,231 @@ int c3d_pm_densify(c3d_pm_session* s, const uint32_t* views, uint32_t n, const c
         HIP_TRY(hipStreamSynchronize(st));
         std::vector<BatchItem> iv(items, items + n);
         unpack(s->host4, iv, out);
+        for (uint32_t i = 0; i < n; ++i)
+            if (out[i].cost) std::memcpy(out[i].cost, s->host1.data() + items[i].off, (size_t)items[i].w * items[i].h * sizeof(float));
+    }
+    if (raw_depth) {
+        std::vector<float> hd((size_t)npix);
+        HIP_TRY(hipMemcpyAsync(hd.data(), s->snap.p, (size_t)npix * sizeof(float), hipMemcpyDeviceToHost, st));
+        HIP_TRY(hipStreamSynchronize(st));
+        for (uint32_t i = 0; i < n; ++i)
+            if (raw_depth[i]) std::memcpy(raw_depth[i], hd.data() + items[i].off, (size_t)items[i].w * items[i].h * sizeof(float));
+    }
+    if (counts) {
+        std::vector<uint8_t> hc((size_t)npix);
+        HIP_TRY(hipMemcpyAsync(hc.data(), s->count.p, (size_t)npix, hipMemcpyDeviceToHost, st));
+        HIP_TRY(hipStreamSynchronize(st));
+        for (uint32_t i = 0; i < n; ++i)
+            if (counts[i]) std::memcpy(counts[i], hc.data() + items[i].off, (size_t)items[i].w * items[i].h);
+    }
+    return C3D_OK;
+}
+
+int c3d_pm_densify_learned(c3d_pm_session* s, c3d_init_net* net, const uint32_t* views, uint32_t n, const c3d_learned_schedule* ls,
+                           c3d_view_state* out, uint8_t* const* counts, float* const* raw_depth, float* const* init_depth,
+                           float* const* init_normal, c3d_timing* timing) {
+    if (!s || !net || !ls || !views || n == 0) return fail(C3D_INVALID_ARGUMENT, "null argument");
+    const nn::NetView nv = nn::net_view(net);
+    if (nv.be != s->be) return fail(C3D_INVALID_ARGUMENT, "the network belongs to another backend");
+    int rc = set_device(s->be);
+    if (rc) return rc;
+    const uint32_t top = (uint32_t)s->nl - 1;
+    rc = check_views(s, top, views, n, out);
+    if (rc) return rc;
+    for (uint32_t i = 0; i < n; ++i)
+        for (uint32_t j = 0; j < i; ++j)
+            if (views[i] == views[j]) return fail(C3D_INVALID_ARGUMENT, "duplicate view in c3d_pm_densify_learned");
+    // quarter-resolution images of the views and of their first two sources
+    std::vector<int> need_q((size_t)s->nv, 0);
+    for (uint32_t i = 0; i < n; ++i) {
+        const int v = (int)views[i];
+        const LevelDesc& L = s->lv[(size_t)v * s->nl + top];
+        if (L.w / 4 > nv.W || L.h / 4 > nv.H || L.w < 4 || L.h < 4)
+            return fail(C3D_INVALID_ARGUMENT, "the network input is smaller than a quarter-resolution view");
+        if (s->vd[v].nsrc < 1) return fail(C3D_INVALID_ARGUMENT, "a view without sources");
+        need_q[v] = 1;
+        for (int j = 0; j < std::min(2, s->vd[v].nsrc); ++j) {
+            // source index from the pair table
+            need_q[(size_t)s->h_pair_src[(size_t)v * MAXSRC + j]] = 1;
+        }
+    }
+    hipStream_t st = s->be->stream;
+    std::vector<size_t> qoff((size_t)s->nv, 0), hoff((size_t)s->nv, 0);
+    size_t qtot = 0, htot = 0;
+    for (int v = 0; v < s->nv; ++v)
+        if (need_q[v]) {
+            const LevelDesc& L = s->lv[(size_t)v * s->nl + top];
+            hoff[v] = htot;
+            htot += (((size_t)(L.w / 2) * (L.h / 2)) + 255) & ~(size_t)255;
+            qoff[v] = qtot;
+            qtot += (((size_t)(L.w / 4) * (L.h / 4)) + 255) & ~(size_t)255;
+        }
+    DevBuf<uint8_t> half, quarter;
+    HIP_TRY(half.reserve(htot));
+    HIP_TRY(quarter.reserve(qtot));
+    // every buffer of the schedule at the top scale
+    std::vector<BatchItem> items;
+    long long npix;
+    int max_nsrc;
+    rc = set_batch(s, views, 0, n, top, items, npix, max_nsrc);
+    if (rc) return rc;
+    HIP_TRY(s->snap.reserve(npix));
+    HIP_TRY(s->plane2.reserve(npix));
+    HIP_TRY(s->need.reserve(n));
+    std::vector<uint8_t> need(n);
+    for (uint32_t i = 0; i < n; ++i) {
+        uint32_t nd = std::min<uint32_t>(s->filter_copy.min_num_consistent, (uint32_t)s->vd[views[i]].nsrc);
+        need[i] = (uint8_t)std::min<uint32_t>(std::max<uint32_t>(nd, 1u), 255u);
+    }
+    HIP_TRY(hipMemcpyAsync(s->need.p, need.data(), n, hipMemcpyHostToDevice, st));
+    std::vector<long long> offs((size_t)s->nv, -1);
+    for (auto& it : items) offs[it.view] = it.off;
+    HIP_TRY(hipMemcpyAsync(s->snap_off.p, offs.data(), offs.size() * sizeof(long long), hipMemcpyHostToDevice, st));
+    HIP_TRY(hipStreamSynchronize(st));
+    s->snap_valid = false;
+    s->pool_used = 0;
+    s->marks_used = 0;
+    MARK(0);
+    // blur3 weights of the pyramid (upsample.rs:47-50), as in c3d_pm_build_pyramid
+    const float e2 = 0x1.152aaap-3f;
+    const float gsum = e2 + 1.0f + e2;
+    const float gw[3] = {e2 / gsum, 1.0f / gsum, e2 / gsum};
+    for (int v = 0; v < s->nv; ++v)
+        if (need_q[v]) {
+            const LevelDesc& L = s->lv[(size_t)v * s->nl + top];
+            HIP_TRY(launch_pyramid_step(s->gray.p + L.goff, L.w, L.h, half.p + hoff[v], gw, st));
+            HIP_TRY(launch_pyramid_step(half.p + hoff[v], L.w / 2, L.h / 2, quarter.p + qoff[v], gw, st));
+        }
+    auto qk = [&](int v, float k[4]) {
+        const LevelDesc& L = s->lv[(size_t)v * s->nl + top];
+        double kk[4] = {L.fx, L.fy, L.cx, L.cy};
+        for (int r = 0; r < 2; ++r) {
+            kk[0] *= 0.5;
+            kk[1] *= 0.5;
+            kk[2] = (kk[2] + 0.5) * 0.5 - 0.5;
+            kk[3] = (kk[3] + 0.5) * 0.5 - 0.5;
+        }
+        for (int j = 0; j < 4; ++j) k[j] = (float)kk[j];
+    };
+    for (uint32_t c0 = 0; c0 < n; c0 += (uint32_t)nv.B) {
+        const uint32_t c1 = std::min<uint32_t>(n, c0 + (uint32_t)nv.B);
+        for (uint32_t i = c0; i < c1; ++i) {
+            const int v = (int)views[i];
+            const LevelDesc& L = s->lv[(size_t)v * s->nl + top];
+            nn::SweepArgs a{};
+            a.ref = quarter.p + qoff[v];
+            a.rw = L.w / 4;
+            a.rh = L.h / 4;
+            qk(v, a.rk);
+            a.nsrc = std::min(2, s->vd[v].nsrc);
+            for (int j = 0; j < a.nsrc; ++j) {
+                const int sv = s->h_pair_src[(size_t)v * MAXSRC + j];
+                const LevelDesc& S = s->lv[(size_t)sv * s->nl + top];
+                a.src[j] = quarter.p + qoff[sv];
+                a.sw[j] = S.w / 4;
+                a.sh[j] = S.h / 4;
+                qk(sv, a.sk[j]);
+                std::memcpy(a.r[j], s->h_pair_rt[(size_t)v * MAXSRC + j].r, sizeof a.r[j]);
+                std::memcpy(a.t[j], s->h_pair_rt[(size_t)v * MAXSRC + j].t, sizeof a.t[j]);
+            }
+            a.dmin = s->vd[v].dmin;
+            a.dmax = s->vd[v].dmax;
+            a.feat = nn::net_features(net, (int)(i - c0));
+            a.H = nv.H;
+            a.W = nv.W;
+            HIP_TRY(nn::sweep(a, st));
+            nn::net_set_range(net, (int)(i - c0), a.dmin, a.dmax);
+        }
+        rc = nn::net_forward(net, (int)(c1 - c0));
+        if (rc) return rc;
+        for (uint32_t i = c0; i < c1; ++i) {
+            const int v = (int)views[i];
+            const LevelDesc& L = s->lv[(size_t)v * s->nl + top];
+            nn::InitPlanesArgs a{};
+            a.out = nn::net_output(net) + (size_t)(i - c0) * nv.H * nv.W * 32;
+            a.H = nv.H;
+            a.W = nv.W;
+            a.qw = L.w / 4;
+            a.qh = L.h / 4;
+            a.dmin = s->vd[v].dmin;
+            a.dmax = s->vd[v].dmax;
+            a.w = L.w;
+            a.h = L.h;
+            a.k[0] = L.fx;
+            a.k[1] = L.fy;
+            a.k[2] = L.cx;
+            a.k[3] = L.cy;
+            a.conf_min = ls->conf_min;
+            a.plane = s->plane.p + items[i].off;
+            HIP_TRY(nn::init_planes(a, st));
+        }
+    }
+    MARK(0);
+    if (init_depth || init_normal) {
+        s->host4.resize((size_t)npix);
+        HIP_TRY(hipMemcpyAsync(s->host4.data(), s->plane.p, (size_t)npix * sizeof(float4), hipMemcpyDeviceToHost, st));
+        HIP_TRY(hipStreamSynchronize(st));
+        for (uint32_t i = 0; i < n; ++i) {
+            const float4* p = s->host4.data() + items[i].off;
+            const size_t np = (size_t)items[i].w * items[i].h;
+            for (size_t q = 0; q < np; ++q) {
+                if (init_depth && init_depth[i]) init_depth[i][q] = p[q].w;
+                if (init_normal && init_normal[i]) {
+                    init_normal[i][3 * q] = p[q].x;
+                    init_normal[i][3 * q + 1] = p[q].y;
+                    init_normal[i][3 * q + 2] = p[q].z;
+                }
+            }
+        }
+    }
+    uint32_t run_id = ls->first_run_id;
+    KParams k = params_for(s, top, false, items, max_nsrc);
+    k.run_id = (int)run_id++;
+    k.random_init = 2;
+    rc = launch_run(s, k, items, s->pm_copy, false, ls->refine_iters, timing != nullptr);
+    if (rc) return rc;
+    MARK(1);
+    for (uint32_t r = 0; r < ls->geometric_rounds; ++r) {
+        HIP_TRY(launch_extract_depth(s->plane.p, s->snap.p, npix, st));
+        KParams g = params_for(s, top, true, items, max_nsrc);
+        g.run_id = (int)run_id++;
+        g.random_init = 0;
+        rc = launch_run(s, g, items, s->pm_copy, true, ls->geometric_iters, timing != nullptr);
+        if (rc) return rc;
+    }
+    MARK(2);
+    HIP_TRY(launch_extract_depth(s->plane.p, s->snap.p, npix, st));
+    KParams f = params_for(s, top, true, items, max_nsrc);
+    f.ntiles = pix_tiles(items.data(), (int)items.size());
+    HIP_TRY(s->ks->filter(f, st));
+    HIP_TRY(launch_threshold(f, s->need.p, s->plane2.p, st));
+    MARK(-1);
+    HIP_TRY(hipEventSynchronize(s->marks[s->marks_used - 1]));
+    if (timing) {
+        // marks: 0 init start, 0 refinement start, 1 geometric, 2 read-out, end
+        float a = 0.f, b = 0.f, c = 0.f, d = 0.f, tot = 0.f;
+        HIP_TRY(hipEventElapsedTime(&a, s->marks[0], s->marks[1]));
+        HIP_TRY(hipEventElapsedTime(&b, s->marks[1], s->marks[2]));
+        HIP_TRY(hipEventElapsedTime(&c, s->marks[2], s->marks[3]));
+        HIP_TRY(hipEventElapsedTime(&d, s->marks[3], s->marks[4]));
+        HIP_TRY(hipEventElapsedTime(&tot, s->marks[0], s->marks[4]));
+        timing->init_ms = a;
+        timing->photometric_ms = b;
+        timing->geometric_ms = c;
+        timing->filter_ms = d;
+        timing->total_ms = tot;
+        rc = pool_sum(s, &timing->half_kernel_ms, &timing->half_launches);
+        if (rc) return rc;
+    }
+    half.release();
+    quarter.release();
+    if (out) {
+        s->host4.resize((size_t)npix);
+        s->host1.resize((size_t)npix);
+        HIP_TRY(hipMemcpyAsync(s->host4.data(), s->plane2.p, (size_t)npix * sizeof(float4), hipMemcpyDeviceToHost, st));
+        HIP_TRY(hipMemcpyAsync(s->host1.data(), s->cost.p, (size_t)npix * sizeof(float), hipMemcpyDeviceToHost, st));
+        HIP_TRY(hipStreamSynchronize(st));
+        unpack(s->host4, items, out);
         for (uint32_t i = 0; i < n; ++i)
             if (out[i].cost) std::memcpy(out[i].cost, s->host1.data() + items[i].off, (size_t)items[i].w * items[i].h * sizeof(float));
     }

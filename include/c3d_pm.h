@@ -161,7 +161,10 @@ typedef struct c3d_depth_snapshot {
     const float* const* maps;
 } c3d_depth_snapshot;
 
-/* Parameters of one run (RunParams, kernel.rs:173-189). */
+/* Parameters of one run (RunParams, kernel.rs:173-189). random_init: 0 = start from the given
+ * planes, 1 = random initialisation of every pixel (the reference's bool), 2 = random
+ * initialisation only where the given depth is not positive (no reference counterpart: the
+ * learned initialiser marks its low-confidence pixels that way). */
 typedef struct c3d_run_params {
     uint32_t level;
     int32_t geometric;
@@ -276,6 +279,7 @@ typedef struct c3d_timing {
     double half_kernel_ms; /* sum over all red-black half-step kernels */
     uint32_t half_launches;
     uint32_t pad0;
+    double init_ms;        /* c3d_pm_densify_learned: features, network and initial planes */
 } c3d_timing;
 
 /* Runs the schedule for `views`; `out`, `counts` and `raw_depth` are at the top scale

@@ -1012,7 +1012,8 @@ static void eval_pixel(const pmo_ctx* c, const launch_t* P, int view, const vsta
     const int pix = y * vs->w + x;
     f3 ray = pixel_ray(R, x, y);
     f4 h = vs->plane[pix];
-    if (mode == 1 && P->random_init) {
+    /* random_init 1: every pixel (patchmatch.cu:668-676); 2: pixels without a positive depth */
+    if (mode == 1 && (P->random_init == 1 || (P->random_init == 2 && !(h.w > 0.f)))) {
         uint64_t st = pmo_rng_state(c->in->seed, kv->key, (uint32_t)P->level, (uint32_t)P->run_id, 0xFFFFu, 0u,
                                     (uint64_t)pix);
         float d = random_depth(&st, kv->depth_min, kv->depth_max);
@@ -1177,7 +1178,7 @@ int pmo_run(pmo_ctx* c, const uint32_t* views, uint32_t n, c3d_view_state* state
         vs.cost = (float*)calloc(np, sizeof(float));
         vs.bview = (uint8_t*)calloc(np, 1);
         vs.prior = use_prior ? (const f4*)s->prior : NULL;
-        if (!p->random_init) states_to_planes(s, vs.plane);
+        if (p->random_init != 1) states_to_planes(s, vs.plane);
         launch_t k;
         memset(&k, 0, sizeof k);
         k.level = (int)p->level;
@@ -1210,7 +1211,7 @@ int pmo_run_rows(pmo_ctx* c, uint32_t view, c3d_view_state* s, const c3d_run_par
     vs.cost = (float*)calloc(np, sizeof(float));
     vs.bview = (uint8_t*)calloc(np, 1);
     vs.prior = NULL;
-    if (!p->random_init) states_to_planes(s, vs.plane);
+    if (p->random_init != 1) states_to_planes(s, vs.plane);
     launch_t k;
     memset(&k, 0, sizeof k);
     k.level = (int)p->level;

@@ -9,16 +9,17 @@ import tempfile
 
 import pytest
 
-from cumulus3d_amd import abi, pm
+from cumulus3d_amd import abi, init_net, pm
 
 ROOT = os.path.dirname(os.path.dirname(os.path.abspath(__file__)))
 HEADER = os.path.join(ROOT, "include", "c3d_pm.h")
+INIT_HEADER = os.path.join(ROOT, "include", "c3d_init.h")
 
 
-def declared_functions():
-    text = open(HEADER).read()
+def declared_functions(header=HEADER, prefix="c3d_pm_"):
+    text = open(header).read()
     text = re.sub(r"/\*.*?\*/", "", text, flags=re.S)
-    return sorted(set(re.findall(r"\b(c3d_pm_\w+)\s*\(", text)))
+    return sorted(set(re.findall(r"\b(" + prefix + r"\w+)\s*\(", text)))
 
 
 def test_header_declares_the_trait_surface():
@@ -33,6 +34,13 @@ def test_library_exports_every_declared_symbol():
     missing = [n for n in declared_functions() if not hasattr(L, n)]
     assert not missing, missing
     assert sorted(pm.EXPORTS) == declared_functions()
+
+
+def test_library_exports_the_initialiser():
+    L = C.CDLL(pm.library_path())
+    names = declared_functions(INIT_HEADER, "c3d_(?:init|pm)_")
+    assert not [n for n in names if not hasattr(L, n)]
+    assert sorted(init_net.EXPORTS) == names
 
 
 STRUCTS = {
@@ -51,8 +59,16 @@ STRUCTS = {
 }
 
 
+STRUCTS.update({
+    "c3d_sweep_source": init_net.c3d_sweep_source,
+    "c3d_sweep_view": init_net.c3d_sweep_view,
+    "c3d_init_train_params": init_net.c3d_init_train_params,
+    "c3d_learned_schedule": init_net.c3d_learned_schedule,
+})
+
+
 def test_struct_layouts_match_the_header():
-    lines = ['#include <stdio.h>', '#include <stddef.h>', f'#include "{HEADER}"', "int main(void) {"]
+    lines = ['#include <stdio.h>', '#include <stddef.h>', f'#include "{HEADER}"', f'#include "{INIT_HEADER}"', "int main(void) {"]
     for name, cls in STRUCTS.items():
         lines.append(f'printf("{name} size %zu\\n", sizeof({name}));')
         for fname, _ in cls._fields_:

@@ -1,0 +1,193 @@
+"""GPU checks of the learned initialiser's hand-written kernels (include/c3d_init.h) against the
+NumPy restatement oracle/nn_oracle.py, through the C ABI.
+
+Bars (bf16 weights and activations, fp32 accumulation; the oracle rounds at the same points and
+sums in float64, so the remaining differences are accumulation order and the occasional bf16
+rounding flip that follows from it):
+  * head outputs of the forward pass within 2 % of the output range of the oracle's;
+  * loss sums within 1e-3 relative, labelled pixel count exact;
+  * every layer's weight gradient within 3 % (relative L2) of the oracle's, cosine > 0.999;
+  * one Adam step equal to the oracle's step on the device's own gradient (1e-6 relative);
+  * plane-sweep features within 2 bf16 ulps of the oracle's;
+  * training on plain-PatchStereo labels lowers the loss.
+"""
+import numpy as np
+import pytest
+
+from cumulus3d_amd import abi, init_net, pm, scenes
+
+pytestmark = pytest.mark.gpu
+
+B, H, W = 2, 16, 24
+TP = init_net.TrainParams(lr=1e-3, w_normal=0.5, w_conf=0.1, conf_tol=0.01)
+
+
+@pytest.fixture(scope="module")
+def setup():
+    from oracle import nn_oracle as nno
+
+    rng = np.random.default_rng(3)
+    be = pm.HipPatchMatch()
+    net = init_net.InitNet(be, B, H, W, seed=11)
+    p = net.get_params()
+    feat = np.zeros((B, H, W, 64), np.float32)
+    feat[..., :33] = rng.uniform(0, 1, (B, H, W, 33))
+    bits = nno.bf16_bits(feat)
+    net.set_features(bits)
+    label = np.zeros((B, H, W, 4), np.float32)
+    label[..., 0] = rng.uniform(20, 60, (B, H, W))
+    label[:, ::5, ::3, 0] = 0.0
+    nrm = rng.normal(size=(B, H, W, 3))
+    nrm[..., 2] = -np.abs(nrm[..., 2]) - 1
+    label[..., 1:] = nrm / np.linalg.norm(nrm, axis=-1, keepdims=True)
+    valid = (W - 3, H - 2)
+    for b in range(B):
+        net.set_labels(b, label[b, ..., 0], label[b, ..., 1:], valid[0], valid[1], 15.0, 120.0)
+    ranges = np.array([[15.0, 120.0]] * B)
+    return be, net, p, nno.bf16(feat), label, valid, ranges
+
+
+def test_forward_matches_oracle(setup):
+    from oracle import nn_oracle as nno
+
+    be, net, p, feat, label, valid, ranges = setup
+    _, _, _, head = net.predict(B, want_head=True)
+    out = nno.Net(p.astype(np.float64), bf=True).forward(feat)["out"][..., :5]
+    err = np.abs(head - out)
+    scale = np.abs(out).max()
+    print(f"head: max |gpu - oracle| {err.max():.3e} of range {scale:.3e}, mean {err.mean():.3e}")
+    assert err.max() <= 0.02 * scale
+    assert err.mean() <= 0.002 * scale
+
+
+def test_gradients_match_oracle(setup):
+    from oracle import nn_oracle as nno
+
+    be, net, p, feat, label, valid, ranges = setup
+    losses, grads = net.train_step(B, TP, step=False, want_grads=True)
+    onet = nno.Net(p.astype(np.float64), bf=True)
+    A = onet.forward(feat)
+    sums, g = onet.loss(A["out"], label, ranges, valid, TP.w_normal, TP.w_conf, TP.conf_tol)
+    ref = onet.backward(A, g)
+    assert losses[3] == sums[3]
+    assert np.allclose(losses[:3], sums[:3], rtol=1e-3)
+    lay, _ = nno.layout()
+    for (name, woff, boff, cin, cout, k, relu) in lay:
+        gw, rw = grads[woff:boff].astype(np.float64), ref[woff:boff]
+        rel = np.linalg.norm(gw - rw) / np.linalg.norm(rw)
+        cos = gw @ rw / (np.linalg.norm(gw) * np.linalg.norm(rw))
+        gb, rb = grads[boff:boff + cout].astype(np.float64), ref[boff:boff + cout]
+        relb = np.linalg.norm(gb - rb) / max(np.linalg.norm(rb), 1e-12)
+        print(f"{name}: weight grad rel {rel:.2e} cos {cos:.6f}, bias rel {relb:.2e}")
+        assert rel < 0.03 and cos > 0.999 and relb < 0.03
+
+
+def test_adam_step_matches_oracle(setup):
+    from oracle import nn_oracle as nno
+
+    be, net, p, feat, label, valid, ranges = setup
+    p0 = net.get_params().astype(np.float64)
+    losses, grads = net.train_step(B, TP, step=True, want_grads=True)
+    p1 = net.get_params().astype(np.float64)
+    want, _, _ = nno.adam(p0, grads.astype(np.float64), np.zeros_like(p0), np.zeros_like(p0), 1, TP.lr, TP.beta1, TP.beta2,
+                          TP.eps, float(losses[3]))
+    assert np.allclose(p1, want, rtol=1e-6, atol=1e-9)
+    net.set_params(p.astype(np.float32))  # back to the initial weights for the other tests
+
+
+def _three_view(width=96, height=64, seed=5):
+    rig = scenes.oblique_rig(3, baseline=6.0)
+    return scenes.make_terrain_scene(rig, width, height, focal=500.0 * width / 640.0, seed=seed)
+
+
+def _sweep_sources(sc, v, others):
+    out = []
+    for s in others:
+        r, t, _ = scenes.pair_geometry(sc, v, s)
+        out.append((sc.views[s].gray, sc.views[s].k, r, t))
+    return out
+
+
+def test_sweep_matches_oracle():
+    from oracle import nn_oracle as nno
+
+    sc = _three_view()
+    be = pm.HipPatchMatch()
+    net = init_net.InitNet(be, 1, 64, 96)
+    keep = []
+    srcs = _sweep_sources(sc, 0, [1, 2])
+    net.features(0, [init_net.sweep_view(sc.views[0].gray, sc.views[0].k, srcs, 20.0, 80.0, keep)])
+    got = net.get_features(0).astype(np.uint32) << 16
+    got = got.view(np.float32).astype(np.float64)
+    ref = nno.sweep(sc.views[0].gray, sc.views[0].k, srcs, 20.0, 80.0, 64, 96)
+    err = np.abs(got - ref)
+    print(f"sweep: max |gpu - oracle| {err.max():.3e}")
+    # bf16 keeps 8 significant bits: within 2 ulps of values <= 1
+    assert err.max() <= 2 * 2.0 ** -8
+    # a pixel near the image border may project just outside in one implementation only
+    assert (err > 2 * 2.0 ** -8).mean() == 0
+
+
+def test_training_lowers_the_loss():
+    """A few hundred Adam steps on 12 terrain views labelled by plain PatchStereo."""
+    nviews, w, h = 12, 96, 64
+    be = pm.HipPatchMatch()
+    net = init_net.InitNet(be, 4, h, w, seed=2)
+    feats, labels, keep = [], [], []
+    for sidx in range(nviews // 3):
+        sc = _three_view(w, h, seed=20 + sidx)
+        keep.append(sc)
+        pmp, fl = abi.fast_profile()
+        inp = scenes.kernel_input(sc, [[1, 2], [0, 2], [0, 1]], [(20.0, 80.0)] * 3, pmp, fl)
+        with be.begin(inp) as s:
+            outs, counts, raws, _ = s.densify([0, 1, 2], abi.Schedule(photometric_iters=6, geometric_rounds=1, geometric_iters=2))
+        for v in range(3):
+            others = [o for o in range(3) if o != v]
+            feats.append(init_net.sweep_view(sc.views[v].gray, sc.views[v].k, _sweep_sources(sc, v, others), 20.0, 80.0, keep))
+            labels.append((outs[v].depth, outs[v].normal))
+    first = last = None
+    for it in range(150):
+        idx = [(it * 4 + j) % nviews for j in range(4)]
+        net.features(0, [feats[i] for i in idx])
+        for j, i in enumerate(idx):
+            net.set_labels(j, labels[i][0], labels[i][1], w, h, 20.0, 80.0)
+        losses, _ = net.train_step(4, TP)
+        tot = (losses[0] + TP.w_normal * losses[1] + TP.w_conf * losses[2]) / max(losses[3], 1)
+        first = tot if first is None else first
+        last = tot
+    print(f"training loss {first:.4f} -> {last:.4f}")
+    assert last < 0.7 * first
+
+
+def test_learned_densify_refines_like_the_oracle():
+    """c3d_pm_densify_learned: the refinement from the network's planes (random initialisation where
+    the confidence is below 0.5) equals the oracle's schedule from the same planes bit for bit."""
+    import oracle
+
+    sc = _three_view(384, 256, seed=31)
+    be = pm.HipPatchMatch()
+    net = init_net.InitNet(be, 3, 64, 96, seed=4)
+    pmp, fl = abi.fast_profile()
+    inp = scenes.kernel_input(sc, [[1, 2], [0, 2], [0, 1]], [(20.0, 80.0)] * 3, pmp, fl)
+    sched = init_net.LearnedSchedule(refine_iters=2, geometric_rounds=1, geometric_iters=2, conf_min=0.5)
+    views = [0, 1, 2]
+    with be.begin(inp) as s:
+        outs, counts, raws, tm, idep, inor = init_net.densify_learned(s, net, views, sched, want_init=True)
+    assert tm.half_launches == 2 * (2 + 2) and tm.init_ms > 0
+    frac = np.mean([(d > 0).mean() for d in idep])
+    print(f"initialised from the network: {frac:.3f} of the pixels")
+    o = oracle.Oracle(inp, threads=16)
+    st = []
+    for v in views:
+        x = abi.ViewState(384, 256)
+        x.depth[...] = idep[v]
+        x.normal[...] = inor[v]
+        st.append(x)
+    o.run(views, st, abi.run_params(0, False, 2, 2, 1, False))
+    o.run(views, st, abi.run_params(0, True, 0, 2, 2, False), abi.DepthSnapshot(1, 0, [x.depth.copy() for x in st]))
+    cnt = o.filter(views, st, abi.DepthSnapshot(2, 0, [x.depth.copy() for x in st]))
+    for v in views:
+        assert np.array_equal(raws[v].view(np.uint32), st[v].depth.view(np.uint32))
+        assert np.array_equal(counts[v], cnt[v])
+        keep = (st[v].depth > 0) & (cnt[v] >= 2)
+        assert np.array_equal(outs[v].depth, np.where(keep, st[v].depth, 0.0).astype(np.float32))
