@@ -11,6 +11,17 @@ schedule), all on the device (c3d_pm_densify).
 
 A step = that whole job. Images are uploaded before the timed region; states stay in HBM.
 value = sum of final W*H over the 7 depth maps / step time (Mpx/s).
+
+Other workloads (--workload):
+  batched48  48 views of a formation flight, resolutions drawn from 960x540 / 1280x720 /
+             1920x1080, 2-8 sources each, one batch per half-step, the whole schedule replayed as a
+             hipGraph.
+  zone108    108 views (36 stations x 3 cameras) at 1920x1080, 4-8 sources each: plain PatchStereo
+             (the default schedule above) against the learned initialiser (network on the quarter-
+             resolution image, 2 refinement iterations, one geometric round of 2), the accuracy of
+             the learned result against the plain one (<= 1 % relative depth error on >= 99 % of
+             the plain method's valid pixels) and the fused clouds of both (median point distance,
+             completeness within 5 cm). value = the learned method's Mpx/s.
 """
 from __future__ import annotations
 
@@ -106,12 +117,119 @@ def cpu_baseline(inp, rows, threads, photometric_iters, view=0):
     return px_iters / dt, dt, f"view {view}, rows {r0}..{r0 + rows} of {w}x{h}, random init + {photometric_iters} photometric iterations"
 
 
+def make_zone_workload(noise, seed, stations=36):
+    """The 108-view zone: 36 stations x 3 cameras at 1920 x 1080, 4-8 sources each (seeded)."""
+    rng = np.random.default_rng(seed)
+    cams = scenes.formation_rig(stations, 3, spacing=8.0)
+    sc = scenes.make_mixed_scene(cams, [(1920, 1080)] * len(cams), seed=seed, noise_sigma=noise, noise_seed=seed + 1)
+    src = scenes.nearest_sources(sc, [int(k) for k in rng.integers(4, 9, len(cams))])
+    pmp, fl = abi.fast_profile()
+    inp = scenes.kernel_input(sc, src, [(15.0, 120.0)] * len(cams), pmp, fl)
+    return sc, inp, src
+
+
+def zone_main(args):
+    """Plain vs learned-init PatchStereo on the 108-view zone, with fusion and cloud comparison."""
+    from cumulus3d_amd import fusion, init_net
+
+    wpath = os.path.join(ROOT, "cumulus3d_amd", "weights", "init_unet.npy")
+    if not os.path.exists(wpath):
+        print(f"{wpath} is missing: train it with python -m cumulus3d_amd.train_init", file=sys.stderr)
+        sys.exit(1)
+    t0 = time.perf_counter()
+    sc, inp, src = make_zone_workload(0.01, seed=5, stations=(args.views or 108) // 3)
+    nviews = len(sc.views)
+    views = list(range(nviews))
+    job_mpx = sum(v.width * v.height for v in sc.views) / 1e6
+    log(f"zone: {nviews} views, {job_mpx:.1f} Mpx rendered in {time.perf_counter() - t0:.1f} s")
+    be = pm.HipPatchMatch()
+    net = init_net.InitNet(be, 8, 272, 480)
+    net.set_params(np.load(wpath, allow_pickle=False).astype(np.float32))
+    plain_s = abi.Schedule(photometric_iters=args.photometric_iters, geometric_rounds=args.geometric_rounds,
+                           geometric_iters=args.geometric_iters)
+    learned_s = init_net.LearnedSchedule(refine_iters=args.refine_iters, geometric_rounds=args.geometric_rounds,
+                                         geometric_iters=args.geometric_iters)
+    res = {}
+    with be.begin(inp) as s:
+        for name, run in [("plain", lambda dl: s.densify(views, plain_s, download=dl)),
+                          ("learned", lambda dl: init_net.densify_learned(s, net, views, learned_s, download=dl))]:
+            for i in range(args.warmup):
+                run(False)
+            device_sync()
+            t = time.perf_counter()
+            tms = []
+            for i in range(args.steps):
+                r = run(False)
+                tms.append(r[3])
+            device_sync()
+            ms = (time.perf_counter() - t) / args.steps * 1e3
+            log(f"{name}: {ms:.1f} ms per job, {job_mpx / (ms / 1e3):.2f} Mpx/s")
+            outs = run(True)
+            res[name] = (ms, outs, tms)
+    # accuracy of the learned maps against the plain ones (on the plain method's valid pixels)
+    ok_d = ok_n = tot = 0
+    for v in views:
+        # raw depth of both methods on the plain method's final (read-out) valid pixels
+        dp, dl = res["plain"][1][2][v], res["learned"][1][2][v]
+        npl, nle = res["plain"][1][0][v].normal, res["learned"][1][0][v].normal
+        m = res["plain"][1][0][v].depth > 0
+        tot += int(m.sum())
+        rel = np.abs(dl[m] - dp[m]) / dp[m]
+        ok_d += int((rel <= 0.01).sum())
+        cosang = np.clip((npl[m] * nle[m]).sum(-1), -1, 1)
+        ok_n += int((np.degrees(np.arccos(cosang)) <= 2.0).sum())
+    frac_d, frac_n = ok_d / max(tot, 1), ok_n / max(tot, 1)
+    log(f"learned vs plain: depth within 1 % on {frac_d:.4f}, normal within 2 deg on {frac_n:.4f} of {tot} px")
+    # fused clouds (min views 3) and their comparison
+    clouds = {}
+    for name in ("plain", "learned"):
+        outs = res[name][1][0]
+        keep = []
+        fv = fusion.fusion_views(sc, [o.depth for o in outs], [o.normal for o in outs], [o.cost for o in outs], src, src, keep)
+        t = time.perf_counter()
+        clouds[name] = fusion.fuse(be, fv, fusion.FusionParams(), int(sum((o.depth > 0).sum() for o in outs)) + 1)
+        log(f"{name} fusion: {len(clouds[name].positions)} points in {time.perf_counter() - t:.1f} s")
+    med, comp = fusion.compare(clouds["learned"], clouds["plain"], 0.05)
+    log(f"clouds: median learned->plain distance {med * 100:.2f} cm, completeness at 5 cm {comp:.4f}")
+    ms_l, ms_p = res["learned"][0], res["plain"][0]
+    line = {
+        "metric": "depth-map Mpx/s at <=1% rel. depth error vs plain PatchStereo; HBM GB/s",
+        "value": round(job_mpx / (ms_l / 1e3), 3),
+        "unit": "Mpx/s",
+        "n_gpus": 1,
+        "steps": args.steps,
+        "warmup": args.warmup,
+        "ms_per_step": round(ms_l, 2),
+        "higher_is_better": True,
+        "scaling": "weak",
+        "vs_baseline": None,
+        "dtype": "f32",
+        "data": "synthetic (seeded fBm terrain, rendered on the host)",
+        "config": {
+            "workload": f"zone: {nviews} views 1920x1080, 4-8 sources, learned initialiser (bf16 U-Net at 480x270) + "
+                        f"{args.refine_iters} refinement iterations + {args.geometric_rounds}x{args.geometric_iters} geometric, "
+                        f"against plain PatchStereo ({args.photometric_iters} + {args.geometric_rounds}x{args.geometric_iters})",
+            "views": nviews,
+            "megapixels": round(job_mpx, 3),
+        },
+        "plain_mpx_s": round(job_mpx / (ms_p / 1e3), 3),
+        "speedup_vs_plain": round(ms_p / ms_l, 3),
+        "accuracy_vs_plain": {"depth_within_1pct": round(frac_d, 5), "normal_within_2deg": round(frac_n, 5),
+                              "gate_depth_99pct": frac_d >= 0.99},
+        "stage_ms_learned": {"init": round(res["learned"][2][-1].init_ms, 2), "photometric": round(res["learned"][2][-1].photometric_ms, 2),
+                             "geometric": round(res["learned"][2][-1].geometric_ms, 2), "filter": round(res["learned"][2][-1].filter_ms, 2)},
+        "clouds": {"plain_points": len(clouds["plain"].positions), "learned_points": len(clouds["learned"].positions),
+                   "median_distance_m": round(med, 5), "completeness_5cm": round(comp, 5)},
+    }
+    print(json.dumps(line), flush=True)
+
+
 def main():
     ap = argparse.ArgumentParser()
     ap.add_argument("--gpus", type=int, default=1)
     ap.add_argument("--steps", type=int, default=3)
     ap.add_argument("--warmup", type=int, default=1)
-    ap.add_argument("--workload", choices=["zone7", "batched48"], default="zone7",
+    ap.add_argument("--workload", choices=["zone7", "batched48", "zone108"], default="zone7",
                     help="zone7: 7 views 1920x1080, 6 sources each (default); batched48: 48 views of uneven "
                          "resolution and source count in one batch, schedule captured in a hipGraph")
     ap.add_argument("--width", type=int, default=1920)
@@ -121,6 +239,7 @@ def main():
     ap.add_argument("--geometric-rounds", type=int, default=1)
     ap.add_argument("--geometric-iters", type=int, default=2)
     ap.add_argument("--graph", type=int, default=None, help="1 = replay the schedule as one hipGraph (default: batched48 only)")
+    ap.add_argument("--refine-iters", type=int, default=2, help="zone108: refinement iterations after the learned init")
     ap.add_argument("--cpu-rows", type=int, default=320)
     ap.add_argument("--no-cpu", action="store_true")
     ap.add_argument("--calibrate-copy", type=int, default=0, metavar="MIB",
@@ -135,6 +254,9 @@ def main():
 
     if args.calibrate_copy:
         copy_calibration(args.calibrate_copy)
+    if args.workload == "zone108":
+        zone_main(args)
+        return
     batched = args.workload == "batched48"
     use_graph = args.graph if args.graph is not None else int(batched)
     t0 = time.perf_counter()

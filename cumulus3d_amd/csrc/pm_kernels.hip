@@ -3,8 +3,8 @@
 //
 // The kernels follow the rules of crates/dense/src/kernel.rs:36-74 as implemented by the reference's
 // crates/cuda/src/kernels/patchmatch.cu (cited per kernel). Work decomposition (MI355X-specific):
-//   * wave64 = 64 active pixels of one row (red-black: every other pixel), 4 rows per 256-thread
-//     workgroup; a workgroup stages its reference patch rows (128 + 2R px wide) in LDS once;
+//   * wave64 = 64 active pixels of one row (red-black: every other pixel), 2 or 4 rows per
+//     workgroup (autotuned); a workgroup stages its reference patch rows (128 + 2R px wide) in LDS once;
 //   * all views of a batch share one flattened 1-D grid of tiles (no empty blocks for smaller
 //     views), remapped so that consecutive tiles run on the same XCD and share its L2;
 //   * the per-thread view weights of the refinement live in LDS, everything else in VGPRs; the
@@ -14,7 +14,8 @@
 
 namespace c3d {
 
-constexpr int HALF_ROWS = 4;     // rows of active pixels per workgroup
+// rows of active pixels per half-step workgroup: a compile-time variant (2 or 4), picked per session
+// by the tile autotuner (c3d_pm_autotune)
 constexpr int HALF_SPAN = 128;   // pixel span of one row of 64 active pixels
 constexpr int PIX_TW = 64;       // eval/filter tiles: 64 x 4 pixels
 constexpr int PIX_TH = 4;
@@ -66,29 +67,30 @@ __device__ __forceinline__ float pair_cost_h(const KParams& P, const RefWinT<WR,
 
 // ================================================================ half step (patchmatch.cu:439-657)
 // GEO: geometric-consistency run (the geometric term is compiled in only there).
-template <int WR, int WSTEP, bool GEO>
-__global__ __launch_bounds__(256) __attribute__((amdgpu_waves_per_eu(2, 2))) void k_half(KParams P) {
+template <int WR, int WSTEP, bool GEO, int ROWS>
+__global__ __launch_bounds__(ROWS * 64) __attribute__((amdgpu_waves_per_eu(2, 2))) void k_half(KParams P) {
+    constexpr int NT = ROWS * 64;
     constexpr int TW = HALF_SPAN + 2 * WR;
-    constexpr int TH = HALF_ROWS + 2 * WR;
+    constexpr int TH = ROWS + 2 * WR;
     __shared__ float tile_s[WR > 0 ? TW * TH : 1];
     // per-thread hypothesis table: the 9 candidates of step 2, then the refinement candidates
-    __shared__ float4 hyp_lds[9 * 256];
+    __shared__ float4 hyp_lds[9 * NT];
     // dynamic LDS: [reference tile (WR = 0 only)] [per-thread view weights]
     extern __shared__ float dyn_lds[];
     const int wr = WR > 0 ? WR : P.wr;
-    const int twd = HALF_SPAN + 2 * wr, thd = HALF_ROWS + 2 * wr;
+    const int twd = HALF_SPAN + 2 * wr, thd = ROWS + 2 * wr;
     float* tile = WR > 0 ? tile_s : dyn_lds;
     float* wv_lds = WR > 0 ? dyn_lds : dyn_lds + twd * thd;
 
     BatchItem it;
     int px0, py0;
-    if (!locate_tile(P, xcd_tile(blockIdx.x, P.ntiles), HALF_SPAN, HALF_ROWS, it, px0, py0)) return;
+    if (!locate_tile(P, xcd_tile(blockIdx.x, P.ntiles), HALF_SPAN, ROWS, it, px0, py0)) return;
     const LevelDesc R = P.lv[it.view * P.nlevels + P.level];
     const int w = it.w, h = it.h;
     {
         const uint8_t* g = P.gray + R.goff;
         const int tx0 = px0 - wr, ty0 = py0 - wr;
-        for (int i = threadIdx.x; i < twd * thd; i += 256) {
+        for (int i = threadIdx.x; i < twd * thd; i += NT) {
             int ty = i / twd, tx = i - ty * twd;
             int gx = tx0 + tx, gy = ty0 + ty;
             tile[i] = (gx >= 0 && gy >= 0 && gx < w && gy < h) ? (float)g[(size_t)gy * w + gx] : 0.0f;
@@ -112,7 +114,7 @@ __global__ __launch_bounds__(256) __attribute__((amdgpu_waves_per_eu(2, 2))) voi
     const f3 ray = pixel_ray(R, x, y);
     const float ifx = 1.0f / R.fx, ify = 1.0f / R.fy;
     float* wv = wv_lds + threadIdx.x;
-    float4* hyp = hyp_lds + threadIdx.x;  // hyp[i * 256]
+    float4* hyp = hyp_lds + threadIdx.x;  // hyp[i * NT]
 
     // 1. candidates: current + best of 8 regions (patchmatch.cu:477-511). The 72 neighbour costs
     //    are loaded unconditionally (out-of-image offsets read the pixel's own cost and are ignored).
@@ -153,7 +155,7 @@ __global__ __launch_bounds__(256) __attribute__((amdgpu_waves_per_eu(2, 2))) voi
                 if (d > 0.f && isfinite(d)) hr = mk4(n, d);
             }
         }
-        hyp[(1 + r) * 256] = hr;
+        hyp[(1 + r) * NT] = hr;
     }
 
     // 2. view selection and aggregation (patchmatch.cu:513-594)
@@ -182,7 +184,7 @@ __global__ __launch_bounds__(256) __attribute__((amdgpu_waves_per_eu(2, 2))) voi
 #pragma nounroll
         for (int i = 0; i < 9; ++i) {
             float gi;
-            const float mi = pair_cost_h<WR, WSTEP, GEO>(P, rw, ref, R, S, pr, lp, ifx, ify, x, y, hyp[i * 256], gi);
+            const float mi = pair_cost_h<WR, WSTEP, GEO>(P, rw, ref, R, S, pr, lp, ifx, ify, x, y, hyp[i * NT], gi);
 #pragma unroll
             for (int q = 0; q < 9; ++q)
                 if (q == i) {
@@ -238,7 +240,7 @@ __global__ __launch_bounds__(256) __attribute__((amdgpu_waves_per_eu(2, 2))) voi
 #pragma unroll
         for (int i = 0; i < 9; ++i) agg[i] = 2.f;
     }
-    float4 best = hyp[bi9 * 256];
+    float4 best = hyp[bi9 * NT];
     float bcost = 2.f;
 #pragma unroll
     for (int q = 0; q < 9; ++q)
@@ -256,22 +258,22 @@ __global__ __launch_bounds__(256) __attribute__((amdgpu_waves_per_eu(2, 2))) voi
     f3 n_p = perturb_normal(st, n0, ray, P.phi);
     float4* cand = hyp;  // the hypothesis table is free again
     cand[0] = mk4(n0, d_r);
-    cand[256] = mk4(n_r, best.w);
-    cand[2 * 256] = mk4(n_r, d_r);
-    cand[3 * 256] = mk4(n0, d_p);
-    cand[4 * 256] = mk4(n_p, best.w);
-    cand[5 * 256] = mk4(n_p, d_p);
+    cand[NT] = mk4(n_r, best.w);
+    cand[2 * NT] = mk4(n_r, d_r);
+    cand[3 * NT] = mk4(n0, d_p);
+    cand[4 * NT] = mk4(n_p, best.w);
+    cand[5 * NT] = mk4(n_p, d_p);
     int ncand = 6;
     if (P.use_prior && rw.var < P.weak_var255) {
         float4 q = P.prior[base + pix];
         if (q.w > 0.f && dot3(xyz(q), ray) < 0.f) {
-            cand[6 * 256] = q;
+            cand[6 * NT] = q;
             ncand = 7;
         }
     }
 #pragma nounroll
     for (int c = 0; c < ncand; ++c) {
-        const float4 hc = cand[c * 256];
+        const float4 hc = cand[c * NT];
         float cc = 2.f;
         if (den > 0.f) {
             float nm = 0.f;
@@ -642,9 +644,9 @@ __global__ void k_pyramid_step(const uint8_t* src, int w, int h, uint8_t* dst, f
 
 // ================================================================ launchers
 
-int half_tiles(const BatchItem* items, int n) {
+int half_tiles(const BatchItem* items, int n, int rows) {
     int t = 0;
-    for (int i = 0; i < n; ++i) t += ((items[i].w + HALF_SPAN - 1) / HALF_SPAN) * ((items[i].h + HALF_ROWS - 1) / HALF_ROWS);
+    for (int i = 0; i < n; ++i) t += ((items[i].w + HALF_SPAN - 1) / HALF_SPAN) * ((items[i].h + rows - 1) / rows);
     return t;
 }
 int pix_tiles(const BatchItem* items, int n) {
@@ -654,15 +656,24 @@ int pix_tiles(const BatchItem* items, int n) {
 }
 
 // dynamic LDS bytes of the any-radius (WR = 0) reference tiles
-static size_t half_tile_bytes(int wr) { return (size_t)(HALF_SPAN + 2 * wr) * (HALF_ROWS + 2 * wr) * sizeof(float); }
+static size_t half_tile_bytes(int wr, int rows) { return (size_t)(HALF_SPAN + 2 * wr) * (rows + 2 * wr) * sizeof(float); }
 static size_t pix_tile_bytes(int wr) { return (size_t)(PIX_TW + 2 * wr) * (PIX_TH + 2 * wr) * sizeof(float); }
 
+template <int WR, int WSTEP, int ROWS>
+static void launch_half_rows(const KParams& P, size_t lds, hipStream_t s) {
+    if (P.geometric) hipLaunchKernelGGL((k_half<WR, WSTEP, true, ROWS>), dim3(P.ntiles), dim3(ROWS * 64), lds, s, P);
+    else hipLaunchKernelGGL((k_half<WR, WSTEP, false, ROWS>), dim3(P.ntiles), dim3(ROWS * 64), lds, s, P);
+}
+
+// P.ntiles must count tiles of `rows` rows (half_tiles(..., rows)); rows is 2 or 4
 template <int WR, int WSTEP>
-static hipError_t launch_half(const KParams& P, hipStream_t s) {
+static hipError_t launch_half(const KParams& P0, int rows, hipStream_t s) {
+    KParams P = P0;
+    P.wv_stride = rows * 64;
     size_t lds = (size_t)P.wv_stride * (P.max_nsrc > 0 ? P.max_nsrc : 1) * sizeof(float);
-    if (WR == 0) lds += half_tile_bytes(P.wr);
-    if (P.geometric) hipLaunchKernelGGL((k_half<WR, WSTEP, true>), dim3(P.ntiles), dim3(256), lds, s, P);
-    else hipLaunchKernelGGL((k_half<WR, WSTEP, false>), dim3(P.ntiles), dim3(256), lds, s, P);
+    if (WR == 0) lds += half_tile_bytes(P.wr, rows);
+    if (rows == 2) launch_half_rows<WR, WSTEP, 2>(P, lds, s);
+    else launch_half_rows<WR, WSTEP, 4>(P, lds, s);
     return hipGetLastError();
 }
 template <int WR, int WSTEP>
@@ -683,7 +694,7 @@ static hipError_t launch_pair_costs(const KParams& P, int view, int slot, int n,
 }
 template <int WR, int WSTEP>
 static const void* half_symbol() {
-    return reinterpret_cast<const void*>(&k_half<WR, WSTEP, false>);
+    return reinterpret_cast<const void*>(&k_half<WR, WSTEP, false, 4>);
 }
 
 template <int WR, int WSTEP>

@@ -173,6 +173,8 @@ struct c3d_pm_session {
     std::vector<BatchItem> h_batch_lv;
     std::vector<long long> h_snap_lv;
     std::vector<uint8_t> h_need;
+    // rows of active pixels per half-step workgroup (options.tile_rows, or c3d_pm_autotune)
+    int half_rows = 4;
     // the captured schedule (options.use_graph) and the key it was captured for
     hipGraphExec_t gexec = nullptr;
     std::vector<uint8_t> gkey;
@@ -352,7 +354,7 @@ int launch_run(c3d_pm_session* s, KParams k, const std::vector<BatchItem>& items
     hipStream_t st = s->be->stream;
     k.ntiles = pix_tiles(items.data(), (int)items.size());
     HIP_TRY(s->ks->eval(k, 1, st));
-    k.ntiles = half_tiles(items.data(), (int)items.size());
+    k.ntiles = half_tiles(items.data(), (int)items.size(), s->half_rows);
     for (uint32_t t = 0; t < iterations; ++t) {
         iteration_constants(pm, geometric, t, k);
         for (int color = 0; color < 2; ++color) {
@@ -365,7 +367,7 @@ int launch_run(c3d_pm_session* s, KParams k, const std::vector<BatchItem>& items
                 }
                 HIP_TRY(hipEventRecord(s->pool[s->pool_used], st));
             }
-            HIP_TRY(s->ks->half(k, st));
+            HIP_TRY(s->ks->half(k, s->half_rows, st));
             if (timed) {
                 HIP_TRY(hipEventRecord(s->pool[s->pool_used + 1], st));
                 s->pool_used += 2;
@@ -602,6 +604,7 @@ int c3d_pm_begin(c3d_pm_backend* be, const c3d_kernel_input* in, c3d_pm_session*
     k.filter_gmax = fl.geom_max_cost;
     s->pm_copy = pm;
     s->filter_copy = fl;
+    s->half_rows = be->opts.tile_rows == 2 ? 2 : 4;
     *out = s;
     return C3D_OK;
 }
@@ -999,6 +1002,7 @@ std::vector<uint8_t> graph_key(c3d_pm_session* s, const uint32_t* views, uint32_
     put(&n, sizeof n);
     put(views, n * sizeof(uint32_t));
     put(sch, sizeof *sch);
+    put(&s->half_rows, sizeof s->half_rows);
     const void* ptrs[] = {s->plane.p, s->cost.p, s->bview.p, s->count.p, s->snap.p, s->plane2.p, s->need.p, s->low.p,
                           s->up.p, s->cinit.p, s->prior.p, s->d_batch_lv.p, s->d_snap_lv.p};
     put(ptrs, sizeof ptrs);
@@ -1335,6 +1339,53 @@ int c3d_pm_densify_learned(c3d_pm_session* s, c3d_init_net* net, const uint32_t*
         HIP_TRY(hipStreamSynchronize(st));
         for (uint32_t i = 0; i < n; ++i)
             if (counts[i]) std::memcpy(counts[i], hc.data() + items[i].off, (size_t)items[i].w * items[i].h);
+    }
+    return C3D_OK;
+}
+
+int c3d_pm_autotune(c3d_pm_session* s, const uint32_t* views, uint32_t n, uint32_t level, uint32_t* rows_out, double* ms_out) {
+    if (!s || !views || n == 0) return fail(C3D_INVALID_ARGUMENT, "null argument");
+    int rc = set_device(s->be);
+    if (rc) return rc;
+    rc = check_views(s, level, views, n, nullptr);
+    if (rc) return rc;
+    hipStream_t st = s->be->stream;
+    std::vector<BatchItem> items;
+    long long npix;
+    int max_nsrc;
+    rc = set_batch(s, views, 0, n, level, items, npix, max_nsrc);
+    if (rc) return rc;
+    // scratch states: a random-init evaluation, then two timed iterations per tile height after one untimed
+    KParams k = params_for(s, level, false, items, max_nsrc);
+    k.run_id = 0x7FFFFFFF;
+    k.random_init = 1;
+    k.ntiles = pix_tiles(items.data(), (int)items.size());
+    HIP_TRY(s->ks->eval(k, 1, st));
+    const int cand[2] = {2, 4};
+    double ms[2] = {0.0, 0.0};
+    for (int c = 0; c < 2; ++c) {
+        KParams h = k;
+        h.random_init = 0;
+        h.ntiles = half_tiles(items.data(), (int)items.size(), cand[c]);
+        for (int it = 0; it < 3; ++it) {
+            iteration_constants(s->pm_copy, false, (uint32_t)it, h);
+            if (it == 1) HIP_TRY(hipEventRecord(s->ev[0], st));
+            for (int color = 0; color < 2; ++color) {
+                h.color = color;
+                HIP_TRY(s->ks->half(h, cand[c], st));
+            }
+        }
+        HIP_TRY(hipEventRecord(s->ev[1], st));
+        HIP_TRY(hipEventSynchronize(s->ev[1]));
+        float t = 0.f;
+        HIP_TRY(hipEventElapsedTime(&t, s->ev[0], s->ev[1]));
+        ms[c] = t;
+    }
+    s->half_rows = ms[0] < ms[1] ? cand[0] : cand[1];
+    if (rows_out) *rows_out = (uint32_t)s->half_rows;
+    if (ms_out) {
+        ms_out[0] = ms[0];
+        ms_out[1] = ms[1];
     }
     return C3D_OK;
 }

@@ -26,7 +26,7 @@ _lock = threading.Lock()
 EXPORTS = [
     "c3d_pm_create", "c3d_pm_name", "c3d_pm_destroy", "c3d_pm_is_available", "c3d_pm_begin", "c3d_pm_run",
     "c3d_pm_evaluate", "c3d_pm_filter", "c3d_pm_upsample", "c3d_pm_median", "c3d_pm_end", "c3d_pm_last_error",
-    "c3d_pm_densify", "c3d_pm_pair_costs", "c3d_pm_num_levels", "c3d_pm_build_pyramid",
+    "c3d_pm_densify", "c3d_pm_pair_costs", "c3d_pm_num_levels", "c3d_pm_build_pyramid", "c3d_pm_autotune",
 ]
 
 
@@ -76,6 +76,8 @@ def load() -> C.CDLL:
                                      C.POINTER(abi.c3d_timing)]
         L.c3d_pm_pair_costs.argtypes = [C.c_void_p, C.c_uint32, C.c_uint32, C.c_uint32, C.c_uint32,
                                         C.POINTER(C.c_int32), C.POINTER(C.c_float), C.POINTER(C.c_float)]
+        L.c3d_pm_autotune.argtypes = [C.c_void_p, u32p, C.c_uint32, C.c_uint32, u32p, C.POINTER(C.c_double)]
+        L.c3d_pm_autotune.restype = C.c_int
         L.c3d_pm_num_levels.argtypes = [C.c_uint32, C.c_uint32, C.c_uint32]
         L.c3d_pm_num_levels.restype = C.c_uint32
         L.c3d_pm_build_pyramid.argtypes = [C.c_void_p, C.POINTER(C.c_uint8), C.c_uint32, C.c_uint32,
@@ -228,6 +230,13 @@ class Session:
         _check(self.L, self.L.c3d_pm_densify(self.h, abi.views_array(views), len(views), C.byref(sch), abi.states_array(outs),
                                              abi.ptr_array(counts, C.c_uint8), abi.ptr_array(raws, C.c_float), C.byref(tm)))
         return outs, counts, raws, tm
+
+    def autotune(self, views: Sequence[int], level: int = 0):
+        """c3d_pm_autotune: the tile height kept for this session and the timings (2 rows, 4 rows)."""
+        rows = C.c_uint32()
+        ms = (C.c_double * 2)()
+        _check(self.L, self.L.c3d_pm_autotune(self.h, abi.views_array(views), len(views), level, C.byref(rows), ms))
+        return int(rows.value), (ms[0], ms[1])
 
     def pair_costs(self, level: int, view: int, slot: int, xy: np.ndarray, planes: np.ndarray) -> np.ndarray:
         """Per-hypothesis photometric cost of explicit planes (c3d_pm_pair_costs)."""

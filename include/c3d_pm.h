@@ -55,7 +55,7 @@ typedef struct c3d_pm_session c3d_pm_session;
 typedef struct c3d_pm_options {
     int64_t batch_pixels;     /* pixel budget of one launch [2,200,000] (batch_pixels) */
     int32_t max_batch;        /* views per launch [64] (max_batch) */
-    int32_t tile_rows;        /* rows of active pixels per workgroup; 0 = tuned default */
+    int32_t tile_rows;        /* rows of active pixels per half-step workgroup (2 or 4); 0 = 4 until c3d_pm_autotune */
     int32_t threads_per_pixel;/* lanes cooperating on one pixel (1, 2 or 4); 0 = tuned default */
     int32_t use_graph;        /* 1 = capture each run's iteration loop in a hipGraph; 0 = plain launches */
     int32_t any_radius;       /* 1 = always the any-radius kernels (weights recomputed per sample), even
@@ -289,6 +289,16 @@ typedef struct c3d_timing {
 int c3d_pm_densify(c3d_pm_session* session, const uint32_t* views, uint32_t num_views,
                    const c3d_schedule* schedule, c3d_view_state* out, uint8_t* const* counts,
                    float* const* raw_depth, c3d_timing* timing);
+
+/* ---------------------------------------------------------------- autotuner (no reference counterpart) */
+
+/* Tile autotuner: times the red-black half step with each compiled tile height (2 and 4 rows of 64
+ * active pixels per workgroup) on `views` at `level` (a random initialisation and two timed
+ * photometric iterations on the session's scratch states, after one untimed) and keeps the faster
+ * for the session's later runs. rows_out (optional) receives the choice, ms_out[2] (optional) the
+ * times of the two iterations for 2 and 4 rows. */
+int c3d_pm_autotune(c3d_pm_session* session, const uint32_t* views, uint32_t num_views, uint32_t level,
+                    uint32_t* rows_out, double* ms_out);
 
 /* ---------------------------------------------------------------- scene preparation */
 

@@ -111,3 +111,19 @@ def test_quality_window_costs_match_fp64():
     err = np.abs(got[both] - ref[both]).max()
     print(f"(5, 1): {both.sum()} costs, max |gpu - fp64| = {err:.2e}")
     assert err < 1e-4
+
+
+def test_tile_rows_identical_and_autotune():
+    """Half-step workgroups of 2 and 4 rows give identical states; the autotuner keeps one of them."""
+    sc, inp = _input(192, 108, 5, 2)
+    outs = []
+    for rows in (2, 4):
+        opts = abi.c3d_pm_options()
+        opts.tile_rows = rows
+        outs.append(_run(pm.HipPatchMatch(0, opts), inp))
+    for x, y in zip(outs[0][0], outs[1][0]):
+        assert _same(x.depth, y.depth) and _same(x.normal, y.normal) and _same(x.cost, y.cost)
+    with pm.HipPatchMatch().begin(inp) as s:
+        rows, ms = s.autotune([0, 1, 2])
+    print(f"autotune: rows {rows}, ms {ms}")
+    assert rows in (2, 4) and min(ms) > 0
