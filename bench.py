@@ -150,7 +150,18 @@ def zone_main(args):
     learned_s = init_net.LearnedSchedule(refine_iters=args.refine_iters, geometric_rounds=args.geometric_rounds,
                                          geometric_iters=args.geometric_iters)
     res = {}
+    tune_rows = None
     with be.begin(inp) as s:
+        if args.autotune:
+            from cumulus3d_amd import autotune
+
+            calib = views[:12]
+            t = time.perf_counter()
+            learned_s, tune_rows = autotune.tune_learned(s, net, calib, plain_s)
+            log(f"schedule autotuner on {len(calib)} views ({time.perf_counter() - t:.1f} s): "
+                + ", ".join(f"{c}: {a:.4f} in {ms:.0f} ms" for c, a, ms in tune_rows)
+                + f" -> refine {learned_s.refine_iters}, geometric {learned_s.geometric_rounds}x{learned_s.geometric_iters}, "
+                  f"confidence {learned_s.conf_min}")
         for name, run in [("plain", lambda dl: s.densify(views, plain_s, download=dl)),
                           ("learned", lambda dl: init_net.densify_learned(s, net, views, learned_s, download=dl))]:
             for i in range(args.warmup):
@@ -166,6 +177,9 @@ def zone_main(args):
             log(f"{name}: {ms:.1f} ms per job, {job_mpx / (ms / 1e3):.2f} Mpx/s")
             outs = run(True)
             res[name] = (ms, outs, tms)
+        # two plain runs that differ only in their random streams: the agreement a method can have
+        reseeded = s.densify(views, abi.Schedule(photometric_iters=args.photometric_iters, geometric_rounds=args.geometric_rounds,
+                                                 geometric_iters=args.geometric_iters, first_run_id=101))
     # accuracy of the learned maps against the plain ones (on the plain method's valid pixels)
     ok_d = ok_n = tot = 0
     for v in views:
@@ -180,6 +194,15 @@ def zone_main(args):
         ok_n += int((np.degrees(np.arccos(cosang)) <= 2.0).sum())
     frac_d, frac_n = ok_d / max(tot, 1), ok_n / max(tot, 1)
     log(f"learned vs plain: depth within 1 % on {frac_d:.4f}, normal within 2 deg on {frac_n:.4f} of {tot} px")
+    rd = rn = 0
+    for v in views:
+        m = res["plain"][1][0][v].depth > 0
+        dp, dr = res["plain"][1][2][v], reseeded[2][v]
+        rd += int((np.abs(dr[m] - dp[m]) <= 0.01 * dp[m]).sum())
+        cosang = np.clip((res["plain"][1][0][v].normal[m] * reseeded[0][v].normal[m]).sum(-1), -1, 1)
+        rn += int((np.degrees(np.arccos(cosang)) <= 2.0).sum())
+    base_d, base_n = rd / max(tot, 1), rn / max(tot, 1)
+    log(f"plain vs plain with other random streams: depth within 1 % on {base_d:.4f}, normal within 2 deg on {base_n:.4f}")
     # fused clouds (min views 3) and their comparison
     clouds = {}
     for name in ("plain", "learned"):
@@ -207,7 +230,7 @@ def zone_main(args):
         "data": "synthetic (seeded fBm terrain, rendered on the host)",
         "config": {
             "workload": f"zone: {nviews} views 1920x1080, 4-8 sources, learned initialiser (bf16 U-Net at 480x270) + "
-                        f"{args.refine_iters} refinement iterations + {args.geometric_rounds}x{args.geometric_iters} geometric, "
+                        f"{learned_s.refine_iters} refinement iterations + {learned_s.geometric_rounds}x{learned_s.geometric_iters} geometric, "
                         f"against plain PatchStereo ({args.photometric_iters} + {args.geometric_rounds}x{args.geometric_iters})",
             "views": nviews,
             "megapixels": round(job_mpx, 3),
@@ -215,7 +238,13 @@ def zone_main(args):
         "plain_mpx_s": round(job_mpx / (ms_p / 1e3), 3),
         "speedup_vs_plain": round(ms_p / ms_l, 3),
         "accuracy_vs_plain": {"depth_within_1pct": round(frac_d, 5), "normal_within_2deg": round(frac_n, 5),
-                              "gate_depth_99pct": frac_d >= 0.99},
+                              "gate_depth_99pct": frac_d >= 0.99,
+                              "plain_reseeded_depth_within_1pct": round(base_d, 5),
+                              "plain_reseeded_normal_within_2deg": round(base_n, 5)},
+        "learned_schedule": {"refine_iters": learned_s.refine_iters, "geometric_rounds": learned_s.geometric_rounds,
+                             "geometric_iters": learned_s.geometric_iters, "conf_min": learned_s.conf_min,
+                             "autotuned_on_views": 12 if tune_rows else 0,
+                             "candidates": [[list(c), round(a, 5), round(ms, 1)] for c, a, ms in tune_rows] if tune_rows else None},
         "stage_ms_learned": {"init": round(res["learned"][2][-1].init_ms, 2), "photometric": round(res["learned"][2][-1].photometric_ms, 2),
                              "geometric": round(res["learned"][2][-1].geometric_ms, 2), "filter": round(res["learned"][2][-1].filter_ms, 2)},
         "clouds": {"plain_points": len(clouds["plain"].positions), "learned_points": len(clouds["learned"].positions),
@@ -240,6 +269,7 @@ def main():
     ap.add_argument("--geometric-iters", type=int, default=2)
     ap.add_argument("--graph", type=int, default=None, help="1 = replay the schedule as one hipGraph (default: batched48 only)")
     ap.add_argument("--refine-iters", type=int, default=2, help="zone108: refinement iterations after the learned init")
+    ap.add_argument("--autotune", type=int, default=1, help="zone108: pick the learned schedule with the autotuner (1) or not (0)")
     ap.add_argument("--cpu-rows", type=int, default=320)
     ap.add_argument("--no-cpu", action="store_true")
     ap.add_argument("--calibrate-copy", type=int, default=0, metavar="MIB",

@@ -14,8 +14,10 @@ import numpy as np
 
 from . import abi, init_net, pm
 
-# (refine iterations, geometric rounds, geometric iterations), cheapest first
-CANDIDATES: List[Tuple[int, int, int]] = [(1, 1, 1), (2, 1, 1), (1, 1, 2), (2, 1, 2), (3, 1, 2), (4, 1, 2), (5, 1, 2)]
+# (refine iterations, geometric rounds, geometric iterations, confidence below which the random
+# initialisation is used), cheapest first
+CANDIDATES: List[Tuple[int, int, int, float]] = [(1, 1, 1, 0.5), (1, 1, 1, 0.25), (1, 1, 1, 0.0), (2, 1, 1, 0.5), (2, 1, 1, 0.25),
+                                                 (2, 1, 2, 0.5), (3, 1, 2, 0.5), (4, 1, 2, 0.5), (5, 1, 2, 0.5)]
 
 
 def depth_agreement(plain_outs, plain_raw, raw) -> float:
@@ -28,20 +30,20 @@ def depth_agreement(plain_outs, plain_raw, raw) -> float:
 
 
 def tune_learned(session: pm.Session, net: init_net.InitNet, views: Sequence[int], plain: abi.Schedule, gate: float = 0.99,
-                 candidates: Sequence[Tuple[int, int, int]] = CANDIDATES):
+                 candidates: Sequence[Tuple[int, int, int, float]] = CANDIDATES):
     """Returns (chosen LearnedSchedule, report rows [(candidate, agreement, ms)]). The first
     candidate (in cost order) whose agreement reaches `gate` wins; if none does, the one with the
     highest agreement."""
     p_outs, _, p_raw, _ = session.densify(views, plain)
     rows = []
     best = None
-    for (ri, gr, gi) in candidates:
-        sch = init_net.LearnedSchedule(refine_iters=ri, geometric_rounds=gr, geometric_iters=gi)
+    for (ri, gr, gi, cm) in candidates:
+        sch = init_net.LearnedSchedule(refine_iters=ri, geometric_rounds=gr, geometric_iters=gi, conf_min=cm)
         t = time.perf_counter()
         outs, _, raw, tm, _, _ = init_net.densify_learned(session, net, views, sch)
         ms = (time.perf_counter() - t) * 1e3
         agree = depth_agreement(p_outs, p_raw, raw)
-        rows.append(((ri, gr, gi), agree, tm.total_ms))
+        rows.append(((ri, gr, gi, cm), agree, tm.total_ms))
         if agree >= gate:
             return sch, rows
         if best is None or agree > best[1]:

@@ -131,14 +131,17 @@ def fuse(backend: pm.HipPatchMatch, views_arr, params: FusionParams, capacity: i
     return Cloud(arrs[0][:m], arrs[1][:m], arrs[2][:m], arrs[3][:m])
 
 
-def compare(a: Cloud, b: Cloud, radius: float = 0.05):
+def compare(a: Cloud, b: Cloud, radius: float = 0.05, max_queries: int = 2_000_000):
     """Median distance from each point of `a` to the nearest point of `b`, and the completeness of
-    `b`: the fraction of its points with a point of `a` within `radius` (m)."""
+    `b`: the fraction of its points with a point of `a` within `radius` (m). Above `max_queries`
+    points the query side is a seeded uniform sample (the trees hold every point)."""
     from scipy.spatial import cKDTree
 
     if len(a.positions) == 0 or len(b.positions) == 0:
         return float("inf"), 0.0
-    da, _ = cKDTree(b.positions.astype(np.float64)).query(a.positions.astype(np.float64), k=1, workers=-1)
-    db, _ = cKDTree(a.positions.astype(np.float64)).query(b.positions.astype(np.float64), k=1, workers=-1,
-                                                          distance_upper_bound=radius)
+    rng = np.random.default_rng(0)
+    qa = a.positions if len(a.positions) <= max_queries else a.positions[rng.choice(len(a.positions), max_queries, replace=False)]
+    qb = b.positions if len(b.positions) <= max_queries else b.positions[rng.choice(len(b.positions), max_queries, replace=False)]
+    da, _ = cKDTree(b.positions.astype(np.float64)).query(qa.astype(np.float64), k=1, workers=-1)
+    db, _ = cKDTree(a.positions.astype(np.float64)).query(qb.astype(np.float64), k=1, workers=-1, distance_upper_bound=radius)
     return float(np.median(da)), float(np.mean(np.isfinite(db)))

@@ -119,7 +119,7 @@ def train(be, data, epochs, batch, lr, seed=1, log_every=50):
 def main():
     ap = argparse.ArgumentParser()
     ap.add_argument("--views", type=int, default=2000)
-    ap.add_argument("--epochs", type=int, default=4)
+    ap.add_argument("--epochs", type=int, default=8)
     ap.add_argument("--batch", type=int, default=8)
     ap.add_argument("--lr", type=float, default=1e-3)
     ap.add_argument("--out", default=os.path.join(os.path.dirname(os.path.abspath(__file__)), "weights", "init_unet.npy"))

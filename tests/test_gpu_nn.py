@@ -191,3 +191,31 @@ def test_learned_densify_refines_like_the_oracle():
         assert np.array_equal(counts[v], cnt[v])
         keep = (st[v].depth > 0) & (cnt[v] >= 2)
         assert np.array_equal(outs[v].depth, np.where(keep, st[v].depth, 0.0).astype(np.float32))
+
+
+def test_trained_network_gains_over_random_init():
+    """The committed weights (cumulus3d_amd/weights/init_unet.npy, trained on plain-PatchStereo
+    labels) against the same short schedule from the random initialisation: on held-out terrain at
+    1920 x 1080, one refinement iteration and one geometric iteration after the network agree with
+    plain PatchStereo (7 iterations + 1 x 2 geometric) on far more pixels than after a random start."""
+    import os
+
+    from cumulus3d_amd import autotune, train_init
+
+    wpath = os.path.join(os.path.dirname(os.path.dirname(os.path.abspath(__file__))), "cumulus3d_amd", "weights", "init_unet.npy")
+    sc = train_init.scene(90001, width=1920, height=1080, focal=500.0)
+    pmp, fl = abi.fast_profile()
+    inp = scenes.kernel_input(sc, [[1, 2], [0, 2], [0, 1]], [(15.0, 120.0)] * 3, pmp, fl)
+    be = pm.HipPatchMatch()
+    net = init_net.InitNet(be, 3, 272, 480)
+    net.set_params(np.load(wpath, allow_pickle=False).astype(np.float32))
+    views = [0, 1, 2]
+    with be.begin(inp) as s:
+        p_outs, _, p_raw, p_tm = s.densify(views, abi.Schedule(photometric_iters=7, geometric_rounds=1, geometric_iters=2))
+        _, _, r_raw, r_tm = s.densify(views, abi.Schedule(photometric_iters=1, geometric_rounds=1, geometric_iters=1))
+        _, _, l_raw, l_tm, _, _ = init_net.densify_learned(s, net, views, init_net.LearnedSchedule(1, 1, 1))
+    learned = autotune.depth_agreement(p_outs, p_raw, l_raw)
+    random = autotune.depth_agreement(p_outs, p_raw, r_raw)
+    print(f"agreement with plain within 1 %: learned {learned:.4f} ({l_tm.total_ms:.1f} ms), "
+          f"random {random:.4f} ({r_tm.total_ms:.1f} ms), plain {p_tm.total_ms:.1f} ms")
+    assert learned > 0.97 and learned > random + 0.1
