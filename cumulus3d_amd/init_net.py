@@ -136,6 +136,7 @@ class InitNet:
         pm._check(self.L, self.L.c3d_init_create(backend.h, max_batch, height, width, seed, C.byref(h)))
         self.h = h
         self.n = int(self.L.c3d_init_num_params(self.h))
+        backend._adopt(self)
 
     def close(self):
         if self.h:

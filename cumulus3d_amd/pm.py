@@ -13,6 +13,7 @@ from __future__ import annotations
 import ctypes as C
 import os
 import threading
+import weakref
 from typing import List, Optional, Sequence
 
 import numpy as np
@@ -113,6 +114,12 @@ class HipPatchMatch:
         h = C.c_void_p()
         _check(self.L, self.L.c3d_pm_create(ordinal, C.byref(options) if options is not None else None, C.byref(h)))
         self.h = h
+        # sessions and networks on this device: released before the device handle, whatever
+        # order the interpreter finalises objects in
+        self._dependents = weakref.WeakSet()
+
+    def _adopt(self, obj):
+        self._dependents.add(obj)
 
     def name(self) -> str:
         return self.L.c3d_pm_name(self.h).decode()
@@ -134,6 +141,8 @@ class HipPatchMatch:
 
     def close(self):
         if self.h:
+            for d in list(self._dependents):
+                d.close()
             self.L.c3d_pm_destroy(self.h)
             self.h = None
 
@@ -154,6 +163,7 @@ class Session:
         h = C.c_void_p()
         _check(self.L, self.L.c3d_pm_begin(backend.h, C.byref(inp.c), C.byref(h)))
         self.h = h
+        backend._adopt(self)
 
     def close(self):
         if self.h:
