@@ -143,6 +143,19 @@ class c3d_run_params(C.Structure):
     ]
 
 
+class c3d_post_params(C.Structure):
+    _fields_ = [
+        ("remove_speckles", C.c_int32),
+        ("speckle_rel_depth", C.c_float),
+        ("speckle_min_area_frac", C.c_double),
+        ("speckle_keep_cost", C.c_float),
+        ("fill_holes", C.c_int32),
+        ("fill_max_frac", C.c_double),
+        ("fill_edge_step", C.c_float),
+        ("fill_max_cost", C.c_float),
+    ]
+
+
 class c3d_schedule(C.Structure):
     _fields_ = [
         ("photometric_iters", C.c_uint32),
@@ -155,6 +168,8 @@ class c3d_schedule(C.Structure):
         ("jbu_sigma_color", C.c_float),
         ("restorer_threshold", C.c_float),
         ("pad0", C.c_uint32),
+        ("post", c3d_post_params),
+        ("post_k", C.POINTER(C.c_double)),
     ]
 
 
@@ -168,10 +183,42 @@ class c3d_timing(C.Structure):
         ("half_launches", C.c_uint32),
         ("pad0", C.c_uint32),
         ("init_ms", C.c_double),
+        ("post_ms", C.c_double),
+        ("post_removed", C.c_uint64),
+        ("post_filled", C.c_uint64),
     ]
 
 
 # ---------------------------------------------------------------- Python-side parameter records
+
+
+@dataclass
+class PostParams:
+    """PostParams (crates/dense/src/postproc.rs:40-73), the reference's defaults; `fast()` adds the
+    fast profile's hole filling (params.rs:456)."""
+
+    remove_speckles: bool = True
+    speckle_rel_depth: float = 0.02
+    speckle_min_area_frac: float = 2e-4
+    speckle_keep_cost: float = 0.2
+    fill_holes: bool = False
+    fill_max_frac: float = 0.01
+    fill_edge_step: float = 0.06
+    fill_max_cost: float = 1.0
+
+    @classmethod
+    def fast(cls) -> "PostParams":
+        return cls(fill_holes=True)
+
+    def c(self) -> c3d_post_params:
+        return c3d_post_params(int(self.remove_speckles), self.speckle_rel_depth, self.speckle_min_area_frac, self.speckle_keep_cost,
+                               int(self.fill_holes), self.fill_max_frac, self.fill_edge_step, self.fill_max_cost)
+
+
+def post_k_array(ks) -> "C.Array":
+    """4 doubles (fx, fy, cx, cy) per view, for c3d_schedule.post_k."""
+    flat = [float(q) for k in ks for q in k]
+    return (C.c_double * max(len(flat), 1))(*flat)
 
 
 @dataclass
@@ -189,6 +236,7 @@ class Schedule:
     jbu_sigma_color: float = 0.04
     restorer_threshold: float = 0.1
     first_run_id: int = 1
+    post: Optional[PostParams] = None  # read-out post-processing (densify.rs:455-462); None = off
 
     @classmethod
     def fast(cls) -> "Schedule":
@@ -199,10 +247,18 @@ class Schedule:
         return cls(photometric_iters=7, photometric_iters_upper=6, geometric_rounds=2, geometric_iters=6,
                    restorer_random_init=True)
 
-    def c(self) -> c3d_schedule:
-        return c3d_schedule(self.photometric_iters, self.photometric_iters_upper, self.geometric_rounds, self.geometric_iters,
-                            self.first_run_id, int(self.restorer_random_init), self.jbu_sigma_spatial, self.jbu_sigma_color,
-                            self.restorer_threshold, 0)
+    def c(self, view_k=None) -> c3d_schedule:
+        """The C struct; `view_k` = top-scale (fx, fy, cx, cy) of each densified view, in order
+        (needed when post.fill_holes)."""
+        sch = c3d_schedule(self.photometric_iters, self.photometric_iters_upper, self.geometric_rounds, self.geometric_iters,
+                           self.first_run_id, int(self.restorer_random_init), self.jbu_sigma_spatial, self.jbu_sigma_color,
+                           self.restorer_threshold, 0)
+        if self.post is not None:
+            sch.post = self.post.c()
+        if view_k is not None:
+            sch._k = post_k_array(view_k)
+            sch.post_k = C.cast(sch._k, C.POINTER(C.c_double))
+        return sch
 
 
 @dataclass
@@ -297,6 +353,7 @@ class ViewData:
     pairs: List[tuple]  # (r[9], t[3], center[3]) float
     depth_min: float
     depth_max: float
+    k: Optional[List[float]] = None  # the image's own fx, fy, cx, cy in double (fill_holes reads view.k)
 
 
 class KernelInput:
@@ -305,6 +362,8 @@ class KernelInput:
     def __init__(self, views: Sequence[ViewData], pm: PmParams, filt: FilterParams, seed: int = DEFAULT_SEED):
         self._keep: list = []
         nl = len(views[0].levels) if views else 0
+        # top-scale intrinsics in double per view (the read-out post-processing's view.k)
+        self.view_k = [[float(q) for q in (v.k if v.k is not None else v.levels[-1].k)] for v in views]
         carr = (c3d_kernel_view * max(len(views), 1))()
         for i, v in enumerate(views):
             if len(v.levels) != nl:

@@ -29,6 +29,7 @@
 #include "nn_kernels.hpp"
 #include "nn_net.hpp"
 #include "pm_kernels.hpp"
+#include "postproc.hpp"
 
 using namespace c3d;
 
@@ -178,6 +179,7 @@ struct c3d_pm_session {
     // the captured schedule (options.use_graph) and the key it was captured for
     hipGraphExec_t gexec = nullptr;
     std::vector<uint8_t> gkey;
+    c3d::PostWork* post = nullptr;  // scratch of the read-out post-processing
 };
 
 namespace {
@@ -640,6 +642,8 @@ void c3d_pm_end(c3d_pm_session* s) {
     for (auto& e : s->pool) (void)hipEventDestroy(e);
     for (auto& e : s->marks) (void)hipEventDestroy(e);
     if (s->gexec) (void)hipGraphExecDestroy(s->gexec);
+    if (s->post) c3d::post_work_free(s->post);
+    s->post = nullptr;
     s->d_batch_lv.release();
     s->d_snap_lv.release();
     delete s;
@@ -995,6 +999,48 @@ int enqueue_densify(c3d_pm_session* s, const DensifyPlan& pl, const c3d_schedule
     return C3D_OK;
 }
 
+// Parameters of the read-out post-processing, checked before any device work.
+int check_post(const c3d_post_params& p, const double* post_k) {
+    int rc = c3d::post_check(p);
+    if (rc) return rc;
+    if (p.fill_holes && !post_k) return fail(C3D_INVALID_ARGUMENT, "fill_holes needs the views' intrinsics (post_k)");
+    return C3D_OK;
+}
+
+// remove_speckles / fill_holes on the thresholded top-scale maps (densify.rs:455-462): planes in
+// plane2, the raw cost in `cost` (read by both, overwritten where a pixel is filled), the views'
+// gray images at the top scale.
+int run_post(c3d_pm_session* s, const c3d_post_params& p, const double* post_k, const BatchItem* items, uint32_t n, c3d_timing* timing) {
+    if (timing) {
+        timing->post_ms = 0.0;
+        timing->post_removed = timing->post_filled = 0;
+    }
+    if (!p.remove_speckles && !p.fill_holes) return C3D_OK;
+    if (!s->post) s->post = c3d::post_work_new();
+    const int top = s->nl - 1;
+    std::vector<c3d::PostView> pv(n);
+    for (uint32_t i = 0; i < n; ++i) {
+        pv[i].off = items[i].off;
+        pv[i].goff = s->lv[(size_t)items[i].view * s->nl + top].goff;
+        pv[i].w = items[i].w;
+        pv[i].h = items[i].h;
+        for (int q = 0; q < 4; ++q) pv[i].k[q] = post_k ? post_k[4 * i + q] : 0.0;
+    }
+    std::vector<unsigned long long> rem(n), fil(n);
+    const auto t0 = std::chrono::steady_clock::now();
+    int rc = c3d::post_run(s->post, pv.data(), (int)n, s->plane2.p, s->cost.p, s->cost.p, s->gray.p, p, s->be->stream, rem.data(),
+                           fil.data());
+    if (rc) return rc;
+    if (timing) {
+        timing->post_ms = std::chrono::duration<double, std::milli>(std::chrono::steady_clock::now() - t0).count();
+        for (uint32_t i = 0; i < n; ++i) {
+            timing->post_removed += rem[i];
+            timing->post_filled += fil[i];
+        }
+    }
+    return C3D_OK;
+}
+
 // Identity of a captured schedule: the views, the schedule and every device address it reads.
 std::vector<uint8_t> graph_key(c3d_pm_session* s, const uint32_t* views, uint32_t n, const c3d_schedule* sch) {
     std::vector<uint8_t> k;
@@ -1026,6 +1072,8 @@ int c3d_pm_densify(c3d_pm_session* s, const uint32_t* views, uint32_t n, const c
             if (views[i] == views[j]) return fail(C3D_INVALID_ARGUMENT, "duplicate view in c3d_pm_densify");
     if (s->nl > 1 && !(sch->jbu_sigma_spatial > 0.f && sch->jbu_sigma_color > 0.f))
         return fail(C3D_INVALID_ARGUMENT, "joint bilateral upsampling needs positive sigmas");
+    rc = check_post(sch->post, sch->post_k);
+    if (rc) return rc;
     hipStream_t st = s->be->stream;
     DensifyPlan pl;
     rc = plan_densify(s, views, n, sch, pl);
@@ -1090,6 +1138,8 @@ int c3d_pm_densify(c3d_pm_session* s, const uint32_t* views, uint32_t n, const c
     }
     const BatchItem* items = s->h_batch_lv.data() + (size_t)top * n;
     const long long npix = pl.npix[top];
+    rc = run_post(s, sch->post, sch->post_k, items, n, timing);
+    if (rc) return rc;
     if (out) {
         s->host4.resize((size_t)npix);
         s->host1.resize((size_t)npix);
@@ -1124,7 +1174,9 @@ int c3d_pm_densify_learned(c3d_pm_session* s, c3d_init_net* net, const uint32_t*
     if (!s || !net || !ls || !views || n == 0) return fail(C3D_INVALID_ARGUMENT, "null argument");
     const nn::NetView nv = nn::net_view(net);
     if (nv.be != s->be) return fail(C3D_INVALID_ARGUMENT, "the network belongs to another backend");
-    int rc = set_device(s->be);
+    int rc = check_post(ls->post, ls->post_k);
+    if (rc) return rc;
+    rc = set_device(s->be);
     if (rc) return rc;
     const uint32_t top = (uint32_t)s->nl - 1;
     rc = check_views(s, top, views, n, out);
@@ -1316,6 +1368,8 @@ int c3d_pm_densify_learned(c3d_pm_session* s, c3d_init_net* net, const uint32_t*
     }
     half.release();
     quarter.release();
+    rc = run_post(s, ls->post, ls->post_k, items.data(), n, timing);
+    if (rc) return rc;
     if (out) {
         s->host4.resize((size_t)npix);
         s->host1.resize((size_t)npix);

@@ -1,0 +1,514 @@
+// postproc.hip — read-out post-processing of depth maps on the MI355X (include/c3d_pm.h,
+// c3d_remove_speckles / c3d_fill_holes and the post stage of c3d_pm_densify):
+// small-component removal (remove_speckles, crates/dense/src/postproc.rs:77-133) and
+// boundary-aware gap filling (fill_holes, postproc.rs:135-204), applied after the threshold as in
+// densify.rs:455-462.
+//
+// Small components. The reference floods each component from its first pixel in raster order;
+// the joining rule |d_j - d_i| <= rel * min(d_i, d_j) is symmetric, so the components are the
+// connected components of the 4-neighbour graph of joining pairs, whatever the visiting order,
+// and erasing one component never changes another. Here they are found by concurrent union-find
+// (atomicMin links, every label points to a smaller index of its component: the root is the
+// component's first pixel, as in the reference), counted with wave-aggregated atomics, and the
+// median cost of every component below the area bound comes from one radix sort of
+// (root, cost in total order) keys: the key at position start + size/2 of a root's run is the
+// element costs[len / 2] of the reference's sorted list.
+//
+// Gap filling reads only pixels that were valid before it (a mask taken first) and writes only
+// pixels that were not, so every pixel is one independent thread, in double precision like the
+// reference.
+#include <hip/hip_runtime.h>
+#include <hipcub/hipcub.hpp>
+
+#include <cmath>
+#include <cstring>
+#include <string>
+#include <vector>
+
+#include "../../include/c3d_pm.h"
+#include "backend.hpp"
+#include "postproc.hpp"
+
+namespace c3d {
+namespace {
+
+constexpr unsigned NONE = 0xFFFFFFFFu;
+
+struct PItem {
+    long long off;   // first pixel in the concatenated arrays
+    long long goff;  // first byte of the view's gray image
+    int w, h;
+    unsigned min_area;  // 0 or 1: no removal (postproc.rs:78-81)
+    int maxd;
+    double k[4];
+};
+
+__device__ __forceinline__ bool item_pixel(const PItem* items, PItem& it, long long& i, int& x, int& y) {
+    it = items[blockIdx.y];
+    const long long q = (long long)blockIdx.x * 256 + threadIdx.x;
+    if (q >= (long long)it.w * it.h) return false;
+    x = (int)(q % it.w);
+    y = (int)(q / it.w);
+    i = it.off + q;
+    return true;
+}
+
+// f32 -> u32 key in IEEE total order (f32::total_cmp) and back
+__device__ __forceinline__ unsigned order_key(float c) {
+    const unsigned u = __float_as_uint(c);
+    return (u & 0x80000000u) ? ~u : (u | 0x80000000u);
+}
+__device__ __forceinline__ float order_value(unsigned k) {
+    return __uint_as_float((k & 0x80000000u) ? (k & 0x7FFFFFFFu) : ~k);
+}
+
+__device__ __forceinline__ unsigned load_label(const unsigned* L, unsigned a) {
+    return __hip_atomic_load(L + a, __ATOMIC_RELAXED, __HIP_MEMORY_SCOPE_AGENT);
+}
+
+__device__ __forceinline__ unsigned find_root(const unsigned* L, unsigned a) {
+    unsigned p = load_label(L, a);
+    while (p != a) {  // labels only point to smaller indices: the walk ends
+        a = p;
+        p = load_label(L, a);
+    }
+    return a;
+}
+
+// Links the trees of a and b: the larger root is pointed at the smaller (Playne and Hawick,
+// "A new algorithm for parallel connected-component labelling on GPUs", IEEE TPDS 2018).
+// Every retry starts from a strictly smaller label, so the loop ends.
+__device__ void unite(unsigned* L, unsigned a, unsigned b) {
+    for (;;) {
+        a = find_root(L, a);
+        b = find_root(L, b);
+        if (a == b) return;
+        if (a < b) {
+            const unsigned old = atomicMin(L + b, a);
+            if (old == b) return;
+            b = old;
+        } else {
+            const unsigned old = atomicMin(L + a, b);
+            if (old == a) return;
+            a = old;
+        }
+    }
+}
+
+// remove_speckles' joining rule (postproc.rs:108-110), in f32 as there
+__device__ __forceinline__ bool joins(float di, float dj, float rel) { return fabsf(dj - di) <= rel * fminf(di, dj); }
+
+// adds `v` for every lane with `on` to *dst with one atomic per wave
+__device__ __forceinline__ void wave_count(bool on, unsigned long long* dst) {
+    const unsigned long long m = __ballot(on);
+    if (m && (threadIdx.x & 63) == (unsigned)__ffsll((long long)m) - 1) atomicAdd(dst, (unsigned long long)__popcll(m));
+}
+
+__global__ void k_clear(unsigned long long* c, int n) {
+    for (int j = (int)threadIdx.x; j < n; j += (int)blockDim.x) c[j] = 0ull;
+}
+
+__global__ __launch_bounds__(256) void k_cc_init(const PItem* items, const float4* plane, unsigned* label, unsigned* size) {
+    PItem it;
+    long long i;
+    int x, y;
+    if (!item_pixel(items, it, i, x, y)) return;
+    label[i] = !(plane[i].w <= 0.0f) ? (unsigned)i : NONE;  // postproc.rs:90: depth <= 0 is no pixel
+    size[i] = 0u;
+}
+
+__global__ __launch_bounds__(256) void k_cc_merge(const PItem* items, const float4* plane, unsigned* label, float rel) {
+    PItem it;
+    long long i;
+    int x, y;
+    if (!item_pixel(items, it, i, x, y)) return;
+    const float di = plane[i].w;
+    if (di <= 0.0f) return;
+    if (x + 1 < it.w) {
+        const float dj = plane[i + 1].w;
+        if (!(dj <= 0.0f) && joins(di, dj, rel)) unite(label, (unsigned)i, (unsigned)(i + 1));
+    }
+    if (y + 1 < it.h) {
+        const float dj = plane[i + it.w].w;
+        if (!(dj <= 0.0f) && joins(di, dj, rel)) unite(label, (unsigned)i, (unsigned)(i + it.w));
+    }
+}
+
+__global__ __launch_bounds__(256) void k_cc_flatten(const PItem* items, unsigned* label, unsigned* size) {
+    PItem it;
+    long long i;
+    int x, y;
+    const bool in = item_pixel(items, it, i, x, y);
+    unsigned r = NONE;
+    if (in && load_label(label, (unsigned)i) != NONE) {
+        r = find_root(label, (unsigned)i);
+        __hip_atomic_store(label + i, r, __ATOMIC_RELAXED, __HIP_MEMORY_SCOPE_AGENT);
+    }
+    // one atomic per distinct root of the wave
+    bool pend = r != NONE;
+    for (;;) {
+        const unsigned long long m = __ballot(pend);
+        if (!m) break;
+        const int lead = __ffsll((long long)m) - 1;
+        const unsigned rl = __shfl(r, lead);
+        const bool same = pend && r == rl;
+        const unsigned long long ms = __ballot(same);
+        if ((int)(threadIdx.x & 63) == lead) atomicAdd(size + rl, (unsigned)__popcll(ms));
+        pend = pend && !same;
+    }
+}
+
+__global__ __launch_bounds__(256) void k_small_keys(const PItem* items, const unsigned* label, const unsigned* size,
+                                                    const float* cost, unsigned long long* keys, unsigned long long* counter) {
+    PItem it;
+    long long i;
+    int x, y;
+    const bool in = item_pixel(items, it, i, x, y);
+    bool small = false;
+    unsigned r = NONE;
+    if (in && it.min_area > 1u) {
+        r = label[i];
+        small = r != NONE && size[r] < it.min_area;
+    }
+    const unsigned long long m = __ballot(small);
+    if (!m) return;
+    const int lane = (int)(threadIdx.x & 63);
+    const int lead = __ffsll((long long)m) - 1;
+    unsigned long long base = 0;
+    if (lane == lead) base = atomicAdd(counter, (unsigned long long)__popcll(m));
+    base = __shfl(base, lead);
+    if (small) {
+        const unsigned long long below = m & ((1ull << lane) - 1ull);
+        keys[base + __popcll(below)] = ((unsigned long long)r << 32) | order_key(cost[i]);
+    }
+}
+
+__global__ __launch_bounds__(256) void k_seg_median(const unsigned long long* keys, long long n, const unsigned* size, float keep,
+                                                    uint8_t* erase) {
+    const long long p = (long long)blockIdx.x * 256 + threadIdx.x;
+    if (p >= n) return;
+    const unsigned r = (unsigned)(keys[p] >> 32);
+    if (p > 0 && (unsigned)(keys[p - 1] >> 32) == r) return;
+    // costs.sort_by(total_cmp); costs[len / 2] > speckle_keep_cost (postproc.rs:119-121)
+    const long long m = p + size[r] / 2;
+    if (m >= n) return;  // the run of r holds size[r] keys
+    const float med = order_value((unsigned)(keys[m] & 0xFFFFFFFFull));
+    erase[r] = med > keep ? 1 : 0;
+}
+
+__global__ __launch_bounds__(256) void k_erase(const PItem* items, float4* plane, const unsigned* label, const unsigned* size,
+                                               const uint8_t* erase, uint8_t* mask, unsigned long long* removed) {
+    PItem it;
+    long long i;
+    int x, y;
+    const bool in = item_pixel(items, it, i, x, y);
+    bool gone = false;
+    if (in) {
+        const unsigned r = label[i];
+        gone = it.min_area > 1u && r != NONE && size[r] < it.min_area && erase[r];
+        if (gone) plane[i] = make_float4(0.f, 0.f, 0.f, 0.f);
+        if (mask) mask[i] = !gone && plane[i].w > 0.0f;
+    }
+    wave_count(gone, removed + blockIdx.y);
+}
+
+__global__ __launch_bounds__(256) void k_mask(const PItem* items, const float4* plane, uint8_t* mask) {
+    PItem it;
+    long long i;
+    int x, y;
+    if (!item_pixel(items, it, i, x, y)) return;
+    mask[i] = plane[i].w > 0.0f;  // postproc.rs:148
+}
+
+// fill_holes (postproc.rs:135-204) for one pixel
+__global__ __launch_bounds__(256) void k_fill(const PItem* items, float4* plane, float* cost, const float* raw_cost, const uint8_t* gray,
+                                              const uint8_t* mask, int step, float max_cost, unsigned long long* filled) {
+    PItem it;
+    long long i;
+    int x, y;
+    const bool in = item_pixel(items, it, i, x, y);
+    bool done = false;
+    if (in && !mask[i] && !(raw_cost[i] > max_cost)) {
+        const uint8_t* g = gray + it.goff;
+        const long long q0 = i - it.off;
+        const int ip = g[q0];
+        bool found = false;
+        int bd = 0, bs = 0;
+        long long bq = 0;
+        const int DX[8] = {1, -1, 0, 0, 1, -1, 1, -1};
+        const int DY[8] = {0, 0, 1, -1, 1, -1, -1, 1};
+#pragma unroll 1
+        for (int dir = 0; dir < 8; ++dir) {
+            int prev = ip;
+            for (int s = 1; s <= it.maxd; ++s) {
+                const int qx = x + DX[dir] * s, qy = y + DY[dir] * s;
+                if (qx < 0 || qy < 0 || qx >= it.w || qy >= it.h) break;
+                const long long q = (long long)qy * it.w + qx;
+                const int iq = g[q];
+                if (abs(iq - prev) > step) break;
+                prev = iq;
+                if (mask[it.off + q]) {
+                    const int dd = abs(iq - ip);
+                    if (!found || dd < bd || (dd == bd && s < bs)) {
+                        found = true;
+                        bd = dd;
+                        bs = s;
+                        bq = q;
+                    }
+                    break;
+                }
+            }
+        }
+        if (found) {
+            const float4 pq = plane[it.off + bq];
+            const double n0 = pq.x, n1 = pq.y, n2 = pq.z;
+            const double qx = (double)(bq % it.w), qy = (double)(bq / it.w);
+            const double rq0 = (qx - it.k[2]) / it.k[0], rq1 = (qy - it.k[3]) / it.k[1];
+            const double rp0 = ((double)x - it.k[2]) / it.k[0], rp1 = ((double)y - it.k[3]) / it.k[1];
+            // plane_transfer (crates/dense/src/math.rs:146-154)
+            const double num = n0 * rq0 + n1 * rq1 + n2 * 1.0;
+            const double den = n0 * rp0 + n1 * rp1 + n2 * 1.0;
+            if (!(den > -1e-6)) {
+                const double d = (double)pq.w * num / den;
+                if (d > 0.0 && isfinite(d)) {
+                    plane[i] = make_float4(pq.x, pq.y, pq.z, (float)d);
+                    cost[i] = cost[it.off + bq];
+                    done = true;
+                }
+            }
+        }
+    }
+    wave_count(done, filled + blockIdx.y);
+}
+
+template <class T>
+hipError_t grow(T*& p, size_t& have, size_t want) {
+    if (want <= have && p) return hipSuccess;
+    if (p) (void)hipFree(p);
+    p = nullptr;
+    have = 0;
+    hipError_t e = hipMalloc((void**)&p, std::max<size_t>(want, 1) * sizeof(T));
+    if (e == hipSuccess) have = std::max<size_t>(want, 1);
+    return e;
+}
+
+}  // namespace
+
+struct PostWork {
+    unsigned* label = nullptr;
+    unsigned* size = nullptr;
+    unsigned long long* keys = nullptr;
+    unsigned long long* keys2 = nullptr;
+    uint8_t* erase = nullptr;
+    uint8_t* mask = nullptr;
+    unsigned long long* counters = nullptr;  // [0] small keys, then removed[n], filled[n]
+    PItem* items = nullptr;
+    void* tmp = nullptr;
+    size_t n_label = 0, n_size = 0, n_keys = 0, n_keys2 = 0, n_erase = 0, n_mask = 0, n_counters = 0, n_items = 0, n_tmp = 0;
+    ~PostWork() {
+        void* all[] = {label, size, keys, keys2, erase, mask, counters, items, tmp};
+        for (void* p : all)
+            if (p) (void)hipFree(p);
+    }
+};
+
+PostWork* post_work_new() { return new PostWork(); }
+void post_work_free(PostWork* w) { delete w; }
+
+#define P_TRY(expr)                                                                                                     \
+    do {                                                                                                                \
+        hipError_t e_ = (expr);                                                                                         \
+        if (e_ != hipSuccess) return set_error(C3D_DEVICE, std::string("HIP: ") + #expr + ": " + hipGetErrorString(e_)); \
+    } while (0)
+
+int post_check(const c3d_post_params& p) {
+    if (p.remove_speckles && !(p.speckle_min_area_frac >= 0.0 && p.speckle_min_area_frac <= 1.0))
+        return set_error(C3D_INVALID_ARGUMENT, "speckle_min_area_frac outside [0, 1]");
+    if (p.fill_holes && !(p.fill_max_frac >= 0.0 && p.fill_max_frac <= 1.0))
+        return set_error(C3D_INVALID_ARGUMENT, "fill_max_frac outside [0, 1]");
+    return C3D_OK;
+}
+
+int post_run(PostWork* w, const PostView* views, int n, float4* plane, float* cost, const float* raw_cost, const uint8_t* gray,
+             const c3d_post_params& p, hipStream_t st, unsigned long long* removed, unsigned long long* filled) {
+    if (n < 1 || n > 65535) return set_error(C3D_INVALID_ARGUMENT, "post-processing of 1..65535 views at a time");
+    int rc = post_check(p);
+    if (rc) return rc;
+    if (!p.remove_speckles && !p.fill_holes) {
+        for (int v = 0; v < n; ++v) {
+            if (removed) removed[v] = 0;
+            if (filled) filled[v] = 0;
+        }
+        return C3D_OK;
+    }
+    std::vector<PItem> it((size_t)n);
+    long long tot = 0, maxpix = 0;
+    for (int v = 0; v < n; ++v) {
+        const PostView& pv = views[v];
+        if (pv.w < 1 || pv.h < 1 || pv.w > 65536 || pv.h > 65536)
+            return set_error(C3D_INVALID_ARGUMENT, "post-processing of an image outside 1..65536 pixels a side");
+        PItem& q = it[(size_t)v];
+        q.off = pv.off;
+        q.goff = pv.goff;
+        q.w = pv.w;
+        q.h = pv.h;
+        const long long np = (long long)pv.w * pv.h;
+        tot = std::max(tot, pv.off + np);
+        maxpix = std::max(maxpix, np);
+        // postproc.rs:78: ceil(w h frac); 0 or 1 = nothing to remove
+        const double ma = std::ceil((double)np * p.speckle_min_area_frac);
+        q.min_area = p.remove_speckles ? (unsigned)std::min(ma, 4294967295.0) : 0u;
+        // postproc.rs:146: max(1, round(sqrt(w^2 + h^2) frac)), the walk cannot leave the image anyway
+        const unsigned long long d2 = (unsigned long long)pv.w * pv.w + (unsigned long long)pv.h * pv.h;
+        const double md = std::round(std::sqrt((double)d2) * p.fill_max_frac);
+        q.maxd = (int)std::max(1.0, std::min(md, 131072.0));
+        std::memcpy(q.k, pv.k, sizeof q.k);
+    }
+    if (tot >= (long long)NONE) return set_error(C3D_INVALID_ARGUMENT, "more than 2^32 - 2 pixels in one post-processing call");
+    P_TRY(grow(w->items, w->n_items, (size_t)n));
+    P_TRY(grow(w->counters, w->n_counters, (size_t)(1 + 2 * n)));
+    P_TRY(grow(w->mask, w->n_mask, (size_t)tot));
+    P_TRY(hipMemcpyAsync(w->items, it.data(), it.size() * sizeof(PItem), hipMemcpyHostToDevice, st));
+    hipLaunchKernelGGL(k_clear, dim3(1), dim3(1024), 0, st, w->counters, 1 + 2 * n);
+    P_TRY(hipGetLastError());
+    unsigned long long* d_removed = w->counters + 1;
+    unsigned long long* d_filled = w->counters + 1 + n;
+    const dim3 grid((unsigned)((maxpix + 255) / 256), (unsigned)n);
+    bool any_small = false;
+    for (const PItem& q : it) any_small = any_small || q.min_area > 1u;
+    if (p.remove_speckles && any_small) {
+        P_TRY(grow(w->label, w->n_label, (size_t)tot));
+        P_TRY(grow(w->size, w->n_size, (size_t)tot));
+        P_TRY(grow(w->keys, w->n_keys, (size_t)tot));
+        P_TRY(grow(w->erase, w->n_erase, (size_t)tot));
+        hipLaunchKernelGGL(k_cc_init, grid, dim3(256), 0, st, w->items, plane, w->label, w->size);
+        hipLaunchKernelGGL(k_cc_merge, grid, dim3(256), 0, st, w->items, plane, w->label, p.speckle_rel_depth);
+        hipLaunchKernelGGL(k_cc_flatten, grid, dim3(256), 0, st, w->items, w->label, w->size);
+        hipLaunchKernelGGL(k_small_keys, grid, dim3(256), 0, st, w->items, w->label, w->size, cost, w->keys, w->counters);
+        P_TRY(hipGetLastError());
+        unsigned long long nsmall = 0;
+        P_TRY(hipMemcpyAsync(&nsmall, w->counters, sizeof nsmall, hipMemcpyDeviceToHost, st));
+        P_TRY(hipStreamSynchronize(st));
+        if (nsmall > (unsigned long long)tot || nsmall > 0x7FFFFFFFull)
+            return set_error(C3D_DEVICE, "post-processing: more small-component pixels than one sort takes");
+        if (nsmall) {
+            P_TRY(grow(w->keys2, w->n_keys2, (size_t)nsmall));
+            size_t bytes = 0;
+            P_TRY(hipcub::DeviceRadixSort::SortKeys(nullptr, bytes, w->keys, w->keys2, (int)nsmall, 0, 64, st));
+            P_TRY(grow((uint8_t*&)w->tmp, w->n_tmp, bytes));
+            P_TRY(hipcub::DeviceRadixSort::SortKeys(w->tmp, bytes, w->keys, w->keys2, (int)nsmall, 0, 64, st));
+            hipLaunchKernelGGL(k_seg_median, dim3((unsigned)((nsmall + 255) / 256)), dim3(256), 0, st, w->keys2, (long long)nsmall,
+                               w->size, p.speckle_keep_cost, w->erase);
+            P_TRY(hipGetLastError());
+        }
+        hipLaunchKernelGGL(k_erase, grid, dim3(256), 0, st, w->items, plane, w->label, w->size, w->erase,
+                           p.fill_holes ? w->mask : nullptr, d_removed);
+        P_TRY(hipGetLastError());
+    } else if (p.fill_holes) {
+        hipLaunchKernelGGL(k_mask, grid, dim3(256), 0, st, w->items, plane, w->mask);
+        P_TRY(hipGetLastError());
+    }
+    if (p.fill_holes) {
+        // postproc.rs:150: (fill_edge_step * 255) as i32, f32 product, truncation (saturating)
+        const float sf = p.fill_edge_step * 255.0f;
+        const int step = sf != sf ? 0 : (sf >= 2147483647.0f ? 2147483647 : (sf <= -2147483648.0f ? (-2147483647 - 1) : (int)sf));
+        hipLaunchKernelGGL(k_fill, grid, dim3(256), 0, st, w->items, plane, cost, raw_cost, gray, w->mask, step, p.fill_max_cost,
+                           d_filled);
+        P_TRY(hipGetLastError());
+    }
+    std::vector<unsigned long long> h((size_t)(1 + 2 * n));
+    P_TRY(hipMemcpyAsync(h.data(), w->counters, h.size() * sizeof(unsigned long long), hipMemcpyDeviceToHost, st));
+    P_TRY(hipStreamSynchronize(st));
+    for (int v = 0; v < n; ++v) {
+        if (removed) removed[v] = h[(size_t)(1 + v)];
+        if (filled) filled[v] = h[(size_t)(1 + n + v)];
+    }
+    return C3D_OK;
+}
+
+}  // namespace c3d
+
+using namespace c3d;
+
+namespace {
+
+// one image through the device post stage: planes packed (nx, ny, nz, d) as in the session
+int post_one(c3d_pm_backend* be, float* depth, float* normal, float* cost, const float* raw_cost, const uint8_t* gray,
+             const double* k, uint32_t w, uint32_t h, const c3d_post_params& p, uint64_t* count) {
+    if (w < 1 || h < 1 || (unsigned long long)w * h >= 0xFFFFFFFFull) return set_error(C3D_INVALID_ARGUMENT, "bad image size");
+    P_TRY(hipSetDevice(be->device));
+    hipStream_t st = be->stream;
+    const size_t np = (size_t)w * h;
+    std::vector<float4> pl(np);
+    for (size_t i = 0; i < np; ++i) pl[i] = make_float4(normal[3 * i], normal[3 * i + 1], normal[3 * i + 2], depth[i]);
+    float4* d_plane = nullptr;
+    float *d_cost = nullptr, *d_raw = nullptr;
+    uint8_t* d_gray = nullptr;
+    struct Free {
+        std::vector<void*> p;
+        ~Free() {
+            for (void* q : p)
+                if (q) (void)hipFree(q);
+        }
+    } fr;
+    P_TRY(hipMalloc((void**)&d_plane, np * sizeof(float4)));
+    fr.p.push_back(d_plane);
+    P_TRY(hipMalloc((void**)&d_cost, np * sizeof(float)));
+    fr.p.push_back(d_cost);
+    if (raw_cost) {
+        P_TRY(hipMalloc((void**)&d_raw, np * sizeof(float)));
+        fr.p.push_back(d_raw);
+        P_TRY(hipMemcpyAsync(d_raw, raw_cost, np * sizeof(float), hipMemcpyHostToDevice, st));
+    }
+    if (gray) {
+        P_TRY(hipMalloc((void**)&d_gray, np));
+        fr.p.push_back(d_gray);
+        P_TRY(hipMemcpyAsync(d_gray, gray, np, hipMemcpyHostToDevice, st));
+    }
+    P_TRY(hipMemcpyAsync(d_plane, pl.data(), np * sizeof(float4), hipMemcpyHostToDevice, st));
+    P_TRY(hipMemcpyAsync(d_cost, cost, np * sizeof(float), hipMemcpyHostToDevice, st));
+    PostView v{};
+    v.off = 0;
+    v.goff = 0;
+    v.w = (int)w;
+    v.h = (int)h;
+    if (k) std::memcpy(v.k, k, sizeof v.k);
+    PostWork work;
+    unsigned long long removed = 0, filled = 0;
+    int rc = post_run(&work, &v, 1, d_plane, d_cost, d_raw, d_gray, p, st, &removed, &filled);
+    if (rc) return rc;
+    P_TRY(hipMemcpyAsync(pl.data(), d_plane, np * sizeof(float4), hipMemcpyDeviceToHost, st));
+    if (p.fill_holes) P_TRY(hipMemcpyAsync(cost, d_cost, np * sizeof(float), hipMemcpyDeviceToHost, st));
+    P_TRY(hipStreamSynchronize(st));
+    for (size_t i = 0; i < np; ++i) {
+        depth[i] = pl[i].w;
+        normal[3 * i] = pl[i].x;
+        normal[3 * i + 1] = pl[i].y;
+        normal[3 * i + 2] = pl[i].z;
+    }
+    if (count) *count = p.remove_speckles ? removed : filled;
+    return C3D_OK;
+}
+
+}  // namespace
+
+extern "C" int c3d_remove_speckles(c3d_pm_backend* be, float* depth, float* normal, const float* cost, uint32_t width,
+                                   uint32_t height, const c3d_post_params* params, uint64_t* removed) {
+    if (!be || !depth || !normal || !cost || !params) return set_error(C3D_INVALID_ARGUMENT, "null argument");
+    c3d_post_params p = *params;
+    p.remove_speckles = 1;
+    p.fill_holes = 0;
+    // the cost is only read: a private copy keeps the caller's buffer const
+    std::vector<float> c(cost, cost + (size_t)width * height);
+    return post_one(be, depth, normal, c.data(), nullptr, nullptr, nullptr, width, height, p, removed);
+}
+
+extern "C" int c3d_fill_holes(c3d_pm_backend* be, float* depth, float* normal, float* cost, const float* raw_cost, const uint8_t* gray,
+                              const double k[4], uint32_t width, uint32_t height, const c3d_post_params* params, uint64_t* filled) {
+    if (!be || !depth || !normal || !cost || !raw_cost || !gray || !k || !params)
+        return set_error(C3D_INVALID_ARGUMENT, "null argument");
+    c3d_post_params p = *params;
+    p.remove_speckles = 0;
+    p.fill_holes = 1;
+    return post_one(be, depth, normal, cost, raw_cost, gray, k, width, height, p, filled);
+}

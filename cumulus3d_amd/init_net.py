@@ -37,7 +37,8 @@ class c3d_init_train_params(C.Structure):
 
 class c3d_learned_schedule(C.Structure):
     _fields_ = [("refine_iters", C.c_uint32), ("geometric_rounds", C.c_uint32), ("geometric_iters", C.c_uint32),
-                ("first_run_id", C.c_uint32), ("conf_min", C.c_float), ("pad0", C.c_uint32)]
+                ("first_run_id", C.c_uint32), ("conf_min", C.c_float), ("pad0", C.c_uint32),
+                ("post", abi.c3d_post_params), ("post_k", C.POINTER(C.c_double))]
 
 
 @dataclass
@@ -50,9 +51,16 @@ class LearnedSchedule:
     geometric_iters: int = 2
     first_run_id: int = 1
     conf_min: float = 0.5
+    post: Optional[abi.PostParams] = None  # read-out post-processing, as in abi.Schedule
 
-    def c(self) -> c3d_learned_schedule:
-        return c3d_learned_schedule(self.refine_iters, self.geometric_rounds, self.geometric_iters, self.first_run_id, self.conf_min, 0)
+    def c(self, view_k=None) -> c3d_learned_schedule:
+        sch = c3d_learned_schedule(self.refine_iters, self.geometric_rounds, self.geometric_iters, self.first_run_id, self.conf_min, 0)
+        if self.post is not None:
+            sch.post = self.post.c()
+        if view_k is not None:
+            sch._k = abi.post_k_array(view_k)
+            sch.post_k = C.cast(sch._k, C.POINTER(C.c_double))
+        return sch
 
 
 @dataclass
@@ -201,7 +209,7 @@ def densify_learned(session: pm.Session, net: InitNet, views: Sequence[int], sch
     L = net.L
     inp = session.inp
     top = inp.num_levels - 1
-    sch = schedule.c()
+    sch = schedule.c([inp.view_k[v] for v in views] if schedule.post is not None else None)
     tm = abi.c3d_timing()
     outs = counts = raws = idep = inor = None
     if download:

@@ -28,6 +28,7 @@ EXPORTS = [
     "c3d_pm_create", "c3d_pm_name", "c3d_pm_destroy", "c3d_pm_is_available", "c3d_pm_begin", "c3d_pm_run",
     "c3d_pm_evaluate", "c3d_pm_filter", "c3d_pm_upsample", "c3d_pm_median", "c3d_pm_end", "c3d_pm_last_error",
     "c3d_pm_densify", "c3d_pm_pair_costs", "c3d_pm_num_levels", "c3d_pm_build_pyramid", "c3d_pm_autotune",
+    "c3d_remove_speckles", "c3d_fill_holes",
 ]
 
 
@@ -84,8 +85,14 @@ def load() -> C.CDLL:
         L.c3d_pm_build_pyramid.argtypes = [C.c_void_p, C.POINTER(C.c_uint8), C.c_uint32, C.c_uint32,
                                            C.POINTER(C.c_double), C.c_uint32, C.POINTER(C.POINTER(C.c_uint8)),
                                            C.POINTER(C.c_float)]
+        fp = C.POINTER(C.c_float)
+        L.c3d_remove_speckles.argtypes = [C.c_void_p, fp, fp, fp, C.c_uint32, C.c_uint32, C.POINTER(abi.c3d_post_params),
+                                          C.POINTER(C.c_uint64)]
+        L.c3d_fill_holes.argtypes = [C.c_void_p, fp, fp, fp, fp, C.POINTER(C.c_uint8), C.POINTER(C.c_double), C.c_uint32,
+                                     C.c_uint32, C.POINTER(abi.c3d_post_params), C.POINTER(C.c_uint64)]
         for n in ("c3d_pm_create", "c3d_pm_begin", "c3d_pm_run", "c3d_pm_evaluate", "c3d_pm_filter", "c3d_pm_upsample",
-                  "c3d_pm_median", "c3d_pm_densify", "c3d_pm_pair_costs", "c3d_pm_is_available", "c3d_pm_build_pyramid"):
+                  "c3d_pm_median", "c3d_pm_densify", "c3d_pm_pair_costs", "c3d_pm_is_available", "c3d_pm_build_pyramid",
+                  "c3d_remove_speckles", "c3d_fill_holes"):
             getattr(L, n).restype = C.c_int
         _libs["engine"] = L
         return L
@@ -94,6 +101,22 @@ def load() -> C.CDLL:
 def _check(L, rc: int):
     if rc != 0:
         raise PmError(rc, L.c3d_pm_last_error().decode(errors="replace"))
+
+
+def _fp(a: np.ndarray):
+    return a.ctypes.data_as(C.POINTER(C.c_float))
+
+
+def _post_shape(depth: np.ndarray, normal: np.ndarray, cost: np.ndarray):
+    """(h, w) of a post-processing call; the arrays are written in place, so they must already be
+    contiguous float32."""
+    if depth.ndim != 2:
+        raise ValueError("depth must be (h, w)")
+    h, w = depth.shape
+    for a, shape in ((depth, (h, w)), (normal, (h, w, 3)), (cost, (h, w))):
+        if a.shape != shape or a.dtype != np.float32 or not a.flags.c_contiguous:
+            raise ValueError(f"expected a contiguous float32 array of shape {shape}")
+    return h, w
 
 
 def num_levels(min_side: int, max_levels: int, min_size: int) -> int:
@@ -138,6 +161,33 @@ class HipPatchMatch:
         _check(self.L, self.L.c3d_pm_build_pyramid(self.h, gray.ctypes.data_as(C.POINTER(C.c_uint8)), w, h, kk, levels,
                                                    abi.ptr_array(outs, C.c_uint8), ko))
         return [abi.LevelData(outs[l], [float(ko[4 * l + j]) for j in range(4)]) for l in range(levels)]
+
+    def remove_speckles(self, depth: np.ndarray, normal: np.ndarray, cost: np.ndarray, params: "abi.PostParams") -> int:
+        """Small-component removal on the device (c3d_remove_speckles; remove_speckles,
+        crates/dense/src/postproc.rs:77-133), depth (h, w) and normal (h, w, 3) float32 in place.
+        Returns the number of erased pixels."""
+        h, w = _post_shape(depth, normal, cost)
+        n = C.c_uint64()
+        p = params.c()
+        _check(self.L, self.L.c3d_remove_speckles(self.h, _fp(depth), _fp(normal), _fp(cost), w, h, C.byref(p), C.byref(n)))
+        return int(n.value)
+
+    def fill_holes(self, depth: np.ndarray, normal: np.ndarray, cost: np.ndarray, raw_cost: np.ndarray, gray: np.ndarray,
+                   k: Sequence[float], params: "abi.PostParams") -> int:
+        """Boundary-aware gap filling on the device (c3d_fill_holes; fill_holes,
+        postproc.rs:135-204): depth, normal and cost in place. Returns the number of filled pixels."""
+        h, w = _post_shape(depth, normal, cost)
+        if raw_cost.shape != (h, w) or raw_cost.dtype != np.float32 or not raw_cost.flags.c_contiguous:
+            raise ValueError("raw_cost must be a contiguous float32 (h, w) array")
+        gray = np.ascontiguousarray(gray, np.uint8)
+        if gray.shape != (h, w):
+            raise ValueError("gray must be (h, w)")
+        n = C.c_uint64()
+        p = params.c()
+        kk = (C.c_double * 4)(*[float(q) for q in k])
+        _check(self.L, self.L.c3d_fill_holes(self.h, _fp(depth), _fp(normal), _fp(cost), _fp(raw_cost),
+                                             gray.ctypes.data_as(C.POINTER(C.c_uint8)), kk, w, h, C.byref(p), C.byref(n)))
+        return int(n.value)
 
     def close(self):
         if self.h:
@@ -224,7 +274,7 @@ class Session:
         """Device-resident compute_depth_maps schedule over every scale (densify.rs:350-454). Returns
         (top-scale states with thresholded depth, counts, raw depth, timing), or (None, None, None,
         timing) without download."""
-        sch = schedule.c()
+        sch = schedule.c([self.inp.view_k[v] for v in views] if schedule.post is not None else None)
         top = self.inp.num_levels - 1
         tm = abi.c3d_timing()
         if not download:

@@ -535,6 +535,7 @@ def kernel_input(
                 pairs=[pair_geometry(scene, v, s) for s in sources[v]],
                 depth_min=float(np.float32(dmin)),
                 depth_max=float(np.float32(dmax)),
+                k=[float(q) for q in view.k],
             )
         )
     pm = abi.PmParams(**{**pm.__dict__, "sigma_spatial": pm.sigma_s()})

@@ -113,6 +113,8 @@ typedef struct c3d_learned_schedule {
     uint32_t first_run_id;
     float conf_min;
     uint32_t pad0;
+    c3d_post_params post;  /* read-out post-processing, as in c3d_schedule */
+    const double* post_k;
 } c3d_learned_schedule;
 
 /* c3d_pm_densify with the learned initialiser in place of the random initialisation, all on the

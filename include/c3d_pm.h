@@ -255,7 +255,25 @@ const char* c3d_pm_last_error(void);
  * (densify.rs:366-372). LevelSchedule (params.rs:476-506) gives: fast profile 6 / 3 iterations, no
  * restorer random init, 2 geometric rounds of 2; quality profile 7 / 6, random init, 2 rounds of 6.
  * There is no host round trip between runs (the reference copies states through the host after
- * every run, patchmatch.rs:785). */
+ * every run, patchmatch.rs:785).
+ * After the threshold, `post` applies the read-out post-processing of densify.rs:455-462 (see
+ * c3d_post_params) to the thresholded maps; fill_holes needs `post_k`. */
+
+/* PostParams (crates/dense/src/postproc.rs:40-73). Reference defaults: remove_speckles 1,
+ * speckle_rel_depth 0.02, speckle_min_area_frac 2e-4, speckle_keep_cost 0.2, fill_holes 0
+ * (1 in the fast profile, params.rs:456), fill_max_frac 0.01, fill_edge_step 0.06,
+ * fill_max_cost 1.0. */
+typedef struct c3d_post_params {
+    int32_t remove_speckles;
+    float speckle_rel_depth;      /* neighbours join when |d_j - d_i| <= rel * min(d_i, d_j) */
+    double speckle_min_area_frac; /* components below ceil(w h frac) pixels ... */
+    float speckle_keep_cost;      /* ... are erased unless their median cost is <= this */
+    int32_t fill_holes;
+    double fill_max_frac;         /* walk at most max(1, round(diagonal * frac)) pixels */
+    float fill_edge_step;         /* stop at a gray step above (int)(step * 255) */
+    float fill_max_cost;          /* fill only pixels whose own raw cost is <= this */
+} c3d_post_params;
+
 typedef struct c3d_schedule {
     uint32_t photometric_iters;       /* scale 0 */
     uint32_t photometric_iters_upper; /* scales > 0 */
@@ -267,6 +285,9 @@ typedef struct c3d_schedule {
     float jbu_sigma_color;    /* DensifyOptions::jbu_sigma_color [0.04] */
     float restorer_threshold; /* DensifyOptions::restorer_threshold [0.1] */
     uint32_t pad0;
+    c3d_post_params post;     /* both flags 0: the thresholded maps are the result */
+    const double* post_k;     /* with post.fill_holes: fx, fy, cx, cy (double) of each view's top
+                                 scale, 4 per entry of `views`, as the scene holds them */
 } c3d_schedule;
 
 /* Per-stage device times of the last c3d_pm_densify call, milliseconds (hipEvent on the session
@@ -280,6 +301,9 @@ typedef struct c3d_timing {
     uint32_t half_launches;
     uint32_t pad0;
     double init_ms;        /* c3d_pm_densify_learned: features, network and initial planes */
+    double post_ms;        /* post-processing (c3d_post_params), host wall time of the stage */
+    uint64_t post_removed; /* pixels erased by remove_speckles, all views */
+    uint64_t post_filled;  /* pixels filled by fill_holes, all views */
 } c3d_timing;
 
 /* Runs the schedule for `views`; `out`, `counts` and `raw_depth` are at the top scale
@@ -289,6 +313,33 @@ typedef struct c3d_timing {
 int c3d_pm_densify(c3d_pm_session* session, const uint32_t* views, uint32_t num_views,
                    const c3d_schedule* schedule, c3d_view_state* out, uint8_t* const* counts,
                    float* const* raw_depth, c3d_timing* timing);
+
+/* ---------------------------------------------------------------- post-processing */
+
+/* Small-component removal of one depth map on the device of `backend` (remove_speckles,
+ * crates/dense/src/postproc.rs:77-133): 4-neighbour components of pixels with depth > 0 joined
+ * when |d_j - d_i| <= speckle_rel_depth * min(d_i, d_j) (f32); a component of fewer than
+ * ceil(width * height * speckle_min_area_frac) pixels whose median cost (element len/2 of its costs
+ * sorted in total order) is above speckle_keep_cost gets depth 0 and normal 0. Nothing happens when
+ * that area bound is 0 or 1. The remove_speckles and fill_holes flags of `params` are not read.
+ * `removed` (may be NULL) receives the number of erased pixels. depth: width*height, normal:
+ * 3 per pixel, cost: width*height. */
+int c3d_remove_speckles(c3d_pm_backend* backend, float* depth, float* normal, const float* cost, uint32_t width,
+                        uint32_t height, const c3d_post_params* params, uint64_t* removed);
+
+/* Boundary-aware gap filling of one depth map on the device of `backend` (fill_holes,
+ * postproc.rs:135-204): every pixel with depth <= 0 (or NaN) and raw_cost <= fill_max_cost walks
+ * the 8 directions (+x, -x, +y, -y, +x+y, -x-y, +x-y, -x+y) for up to
+ * max(1, round(sqrt(w^2 + h^2) * fill_max_frac)) steps, stopping at the border or where the gray
+ * value changes by more than (int)(fill_edge_step * 255) from the previous step; the first pixel
+ * with depth > 0 met in a direction is a candidate, and the candidate of least gray difference to
+ * the pixel (then least distance, then earliest direction) lends its plane: the depth moves onto
+ * the pixel's ray (plane_transfer, math.rs:146-154, double precision with `k` = fx, fy, cx, cy),
+ * the normal and the cost are copied. Only pixels valid before the call are read. `filled` (may be
+ * NULL) receives the number of filled pixels. The flags of `params` are not read. */
+int c3d_fill_holes(c3d_pm_backend* backend, float* depth, float* normal, float* cost, const float* raw_cost,
+                   const uint8_t* gray, const double k[4], uint32_t width, uint32_t height,
+                   const c3d_post_params* params, uint64_t* filled);
 
 /* ---------------------------------------------------------------- autotuner (no reference counterpart) */
 

@@ -104,7 +104,11 @@ def load_library(path: str):
         L.pmo_fuse_consistency.argtypes = [C.POINTER(_fusion.c3d_fusion_view), C.c_uint32, C.POINTER(_fusion.c3d_fusion_params),
                                            C.POINTER(_fusion.c3d_point_cloud)]
         L.pmo_fuse_consistency.restype = C.c_int
-        for n in ("pmo_run", "pmo_evaluate", "pmo_filter", "pmo_upsample", "pmo_median", "pmo_densify", "pmo_run_rows"):
+        L.pmo_remove_speckles.argtypes = [fp, fp, fp, C.c_uint32, C.c_uint32, C.POINTER(abi.c3d_post_params), C.POINTER(C.c_uint64)]
+        L.pmo_fill_holes.argtypes = [fp, fp, fp, fp, C.POINTER(C.c_uint8), dp, C.c_uint32, C.c_uint32, C.POINTER(abi.c3d_post_params),
+                                     C.POINTER(C.c_uint64)]
+        for n in ("pmo_run", "pmo_evaluate", "pmo_filter", "pmo_upsample", "pmo_median", "pmo_densify", "pmo_run_rows",
+                  "pmo_remove_speckles", "pmo_fill_holes"):
             getattr(L, n).restype = C.c_int
         return L
 
@@ -134,6 +138,30 @@ def fuse_consistency(views_arr, params, capacity: int):
         raise OracleError(f"fusion capacity {capacity} < {pc.count}")
     m = int(pc.count)
     return _fusion.Cloud(arrs[0][:m], arrs[1][:m], arrs[2][:m], arrs[3][:m])
+
+
+def remove_speckles(depth, normal, cost, params):
+    """remove_speckles (oracle/post_oracle.c; postproc.rs:77-133) in place; returns the count."""
+    h, w = depth.shape
+    n = C.c_uint64()
+    p = params.c()
+    fp = C.POINTER(C.c_float)
+    lib().pmo_remove_speckles(depth.ctypes.data_as(fp), normal.ctypes.data_as(fp), cost.ctypes.data_as(fp), w, h, C.byref(p),
+                              C.byref(n))
+    return int(n.value)
+
+
+def fill_holes(depth, normal, cost, raw_cost, gray, k, params):
+    """fill_holes (oracle/post_oracle.c; postproc.rs:135-204) in place; returns the count."""
+    h, w = depth.shape
+    n = C.c_uint64()
+    p = params.c()
+    fp = C.POINTER(C.c_float)
+    gray = np.ascontiguousarray(gray, np.uint8)
+    lib().pmo_fill_holes(depth.ctypes.data_as(fp), normal.ctypes.data_as(fp), cost.ctypes.data_as(fp), raw_cost.ctypes.data_as(fp),
+                         gray.ctypes.data_as(C.POINTER(C.c_uint8)), (C.c_double * 4)(*[float(q) for q in k]), w, h, C.byref(p),
+                         C.byref(n))
+    return int(n.value)
 
 
 class OracleError(RuntimeError):
@@ -204,7 +232,7 @@ class Oracle:
     def densify(self, views, schedule: abi.Schedule):
         """compute_depth_maps over every scale (densify.rs:350-454): returns top-scale (states with
         thresholded depth, counts, raw depth)."""
-        sch = schedule.c()
+        sch = schedule.c([self.inp.view_k[v] for v in views] if schedule.post is not None else None)
         top = self.inp.num_levels - 1
         outs, counts, raws = [], [], []
         for v in views:

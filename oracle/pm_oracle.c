@@ -1601,6 +1601,19 @@ int pmo_densify(pmo_ctx* c, const uint32_t* views, uint32_t n, const c3d_schedul
             out[i].normal[3 * q + 2] = keep ? st[i].normal[3 * q + 2] : 0.f;
             if (out[i].cost) out[i].cost[q] = st[i].cost[q];
         }
+        /* read-out post-processing (densify.rs:455-462): speckles judged on the raw cost, gaps
+         * filled from the raw cost with the top-scale gray image */
+        float* cw = out[i].cost ? out[i].cost : (float*)malloc(np * sizeof(float));
+        if (!out[i].cost) memcpy(cw, st[i].cost, np * sizeof(float));
+        if (sch->post.remove_speckles)
+            pmo_remove_speckles(out[i].depth, out[i].normal, st[i].cost, st[i].width, st[i].height, &sch->post, NULL);
+        if (sch->post.fill_holes) {
+            if (!sch->post_k) rc = C3D_INVALID_ARGUMENT;
+            else
+                pmo_fill_holes(out[i].depth, out[i].normal, cw, st[i].cost, c->in->views[views[i]].levels[top].gray,
+                               sch->post_k + 4 * i, st[i].width, st[i].height, &sch->post, NULL);
+        }
+        if (!out[i].cost) free(cw);
     }
     for (uint32_t i = 0; i < n; ++i)
         if (!counts && cnt[i]) free(cnt[i]);

@@ -86,6 +86,12 @@ int pmo_upsample(pmo_ctx* ctx, uint32_t level, const uint32_t* views, uint32_t n
                  const c3d_view_state* low, float sigma_s, float sigma_c, c3d_view_state* out);
 int pmo_median(pmo_ctx* ctx, uint32_t level, const uint32_t* views, uint32_t n,
                const c3d_view_state* in, c3d_view_state* out);
+/* read-out post-processing (post_oracle.c; postproc.rs:77-204) */
+int pmo_remove_speckles(float* depth, float* normal, const float* cost, uint32_t w, uint32_t h,
+                        const c3d_post_params* p, uint64_t* removed);
+int pmo_fill_holes(float* depth, float* normal, float* cost, const float* raw_cost, const uint8_t* gray,
+                   const double k[4], uint32_t w, uint32_t h, const c3d_post_params* p, uint64_t* filled);
+
 int pmo_densify(pmo_ctx* ctx, const uint32_t* views, uint32_t n, const c3d_schedule* sch,
                 c3d_view_state* out, uint8_t* const* counts, float* const* raw_depth);
 

@@ -16,7 +16,7 @@ HEADER = os.path.join(ROOT, "include", "c3d_pm.h")
 INIT_HEADER = os.path.join(ROOT, "include", "c3d_init.h")
 
 
-def declared_functions(header=HEADER, prefix="c3d_pm_"):
+def declared_functions(header=HEADER, prefix="c3d_"):
     text = open(header).read()
     text = re.sub(r"/\*.*?\*/", "", text, flags=re.S)
     return sorted(set(re.findall(r"\b(" + prefix + r"\w+)\s*\(", text)))
@@ -54,6 +54,7 @@ STRUCTS = {
     "c3d_view_state": abi.c3d_view_state,
     "c3d_depth_snapshot": abi.c3d_depth_snapshot,
     "c3d_run_params": abi.c3d_run_params,
+    "c3d_post_params": abi.c3d_post_params,
     "c3d_schedule": abi.c3d_schedule,
     "c3d_timing": abi.c3d_timing,
 }
