@@ -43,7 +43,7 @@ def _engine(name: str, defines: list[str]) -> str:
     os.makedirs(os.path.join(LIB, "obj"), exist_ok=True)
     objs = []
     jobs = []
-    for src in ("pm_kernels.hip", "pm_session.cpp", "nn_kernels.hip", "nn_net.cpp", "fusion.hip", "postproc.hip"):
+    for src in ("pm_kernels.hip", "pm_session.cpp", "nn_kernels.hip", "nn_net.cpp", "fusion.hip", "postproc.hip", "neighbors.hip"):
         obj = os.path.join(LIB, "obj", f"{name}_{os.path.splitext(src)[0]}.o")
         jobs.append([HIPCC, *HIP_FLAGS, *defines, "-c", os.path.join(CSRC, src), "-o", obj])
         objs.append(obj)

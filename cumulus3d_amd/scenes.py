@@ -5,8 +5,8 @@
 * :func:`pair_geometry`: reference->source relative geometry (crates/dense/src/densify.rs:194-205).
 * :func:`make_synth_scene`: the reference's accuracy fixture, a textured floor with two boxes seen by a
   3x3 camera grid with ground-truth depth and normals (crates/dense/src/synthetic.rs:36-292).
-* :func:`select_sources`, :func:`depth_ranges`: neighbour selection and depth ranges from sparse
-  points (crates/dense/src/neighbors.rs:39-187).
+* sparse points (ScenePoint) for the neighbour selection and depth ranges of neighbors.py
+  (crates/dense/src/neighbors.rs:39-189).
 * :func:`make_terrain_scene`: seeded fBm terrain imaged by oblique pinhole cameras (no reference
   counterpart; rendered by csrc/terrain.c).
 * :func:`kernel_input`: builds the C-ABI ``KernelInput`` (key = FNV-1a of the image name,
@@ -301,78 +301,6 @@ def make_synth_scene(cfg: SynthConfig = SynthConfig()) -> DenseScene:
         if len(obs) >= 2:
             points.append(ScenePoint(np.asarray(p), obs))
     return DenseScene(views, points, depths, normals)
-
-
-# ---------------------------------------------------------------- neighbours (neighbors.rs)
-
-
-def select_sources(scene: DenseScene, max_views: int = 10, min_angle_deg: float = 1.0) -> List[List[int]]:
-    """PairStats::select_all (neighbors.rs:39-86,150-155): per view, the views whose 75th-percentile
-    triangulation angle is at least the bound, by descending shared-point count (ascending index on ties)."""
-    n = len(scene.views)
-    centers = [v.center() for v in scene.views]
-    stats = {}
-    for p in scene.points:
-        x = p.xyz
-        for i in range(len(p.views)):
-            for j in range(i):
-                a, b = p.views[i], p.views[j]
-                if a == b:
-                    continue
-                u, v = centers[a] - x, centers[b] - x
-                nu, nv = np.linalg.norm(u), np.linalg.norm(v)
-                if nu == 0 or nv == 0:
-                    ang = 0.0
-                else:
-                    cc = float(np.clip(np.dot(u, v) / (nu * nv), -1.0, 1.0))
-                    ang = math.acos(cc)
-                    ang = min(ang, math.pi - ang)
-                e = stats.setdefault((min(a, b), max(a, b)), [0, []])
-                e[0] += 1
-                e[1].append(float(np.float32(ang)))
-    amin = math.radians(min_angle_deg)
-    out = []
-    for r in range(n):
-        cands = []
-        for o in range(n):
-            if o == r or (min(r, o), max(r, o)) not in stats:
-                continue
-            cnt, angs = stats[(min(r, o), max(r, o))]
-            if amin > 0.0:
-                a = sorted(angs)
-                pos = 0.75 * (len(a) - 1)
-                lo = int(math.floor(pos))
-                hi = min(lo + 1, len(a) - 1)
-                q = a[lo] + (a[hi] - a[lo]) * (pos - lo)
-                if q < amin:
-                    continue
-            cands.append((cnt, o))
-        cands.sort(key=lambda c: (-c[0], c[1]))
-        out.append([o for _, o in cands[:max_views]])
-    return out
-
-
-def depth_range_of(values: Sequence[float]):
-    """neighbors.rs:172-187: 1%/99% order statistics times 0.75/1.25."""
-    l = sorted(values)
-    if not l:
-        return None
-    n = len(l)
-    lo = l[min(int(math.floor(n * 0.01)), n - 1)]
-    hi = l[min(int(math.floor(n * 0.99)), n - 1)]
-    return 0.75 * lo, 1.25 * hi
-
-
-def depth_ranges(scene: DenseScene):
-    """neighbors.rs:153-170."""
-    lists = [[] for _ in scene.views]
-    for p in scene.points:
-        for v in p.views:
-            vw = scene.views[v]
-            d = float(vw.r[2] @ p.xyz + vw.t[2])
-            if d > 0.0:
-                lists[v].append(d)
-    return [depth_range_of(l) for l in lists]
 
 
 # ---------------------------------------------------------------- terrain scenes

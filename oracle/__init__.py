@@ -107,8 +107,12 @@ def load_library(path: str):
         L.pmo_remove_speckles.argtypes = [fp, fp, fp, C.c_uint32, C.c_uint32, C.POINTER(abi.c3d_post_params), C.POINTER(C.c_uint64)]
         L.pmo_fill_holes.argtypes = [fp, fp, fp, fp, C.POINTER(C.c_uint8), dp, C.c_uint32, C.c_uint32, C.POINTER(abi.c3d_post_params),
                                      C.POINTER(C.c_uint64)]
+        from cumulus3d_amd.neighbors import c3d_sparse_scene
+
+        L.pmo_pair_stats.argtypes = [C.POINTER(c3d_sparse_scene), C.POINTER(C.c_uint32), dp]
+        L.pmo_depth_ranges.argtypes = [C.POINTER(c3d_sparse_scene), dp, dp, C.POINTER(C.c_uint8)]
         for n in ("pmo_run", "pmo_evaluate", "pmo_filter", "pmo_upsample", "pmo_median", "pmo_densify", "pmo_run_rows",
-                  "pmo_remove_speckles", "pmo_fill_holes"):
+                  "pmo_remove_speckles", "pmo_fill_holes", "pmo_pair_stats", "pmo_depth_ranges"):
             getattr(L, n).restype = C.c_int
         return L
 
@@ -138,6 +142,46 @@ def fuse_consistency(views_arr, params, capacity: int):
         raise OracleError(f"fusion capacity {capacity} < {pc.count}")
     m = int(pc.count)
     return _fusion.Cloud(arrs[0][:m], arrs[1][:m], arrs[2][:m], arrs[3][:m])
+
+
+def pair_stats(scene):
+    """PairStats::new (oracle/scene_oracle.c; neighbors.rs:47-80): (count, q75), n x n."""
+    from cumulus3d_amd.neighbors import SparseScene
+
+    s = scene if isinstance(scene, SparseScene) else SparseScene(scene)
+    count = np.zeros((s.n, s.n), np.uint32)
+    q75 = np.zeros((s.n, s.n), np.float64)
+    lib().pmo_pair_stats(C.byref(s.c), count.ctypes.data_as(C.POINTER(C.c_uint32)), q75.ctypes.data_as(C.POINTER(C.c_double)))
+    return count, q75
+
+
+def select(count, q75, r, max_views, min_angle_rad):
+    """PairStats::select (neighbors.rs:83-103), restated."""
+    cands = [(int(count[r, o]), o) for o in range(count.shape[0])
+             if o != r and count[r, o] > 0 and not (min_angle_rad > 0.0 and q75[r, o] < min_angle_rad)]
+    cands.sort(key=lambda c: (-c[0], c[1]))
+    return [o for _, o in cands[:max_views]]
+
+
+def select_sources(scene, max_views=10, min_angle_deg=1.0):
+    """select_all (neighbors.rs:154-157) over the oracle's pair statistics."""
+    import math
+
+    count, q75 = pair_stats(scene)
+    return [select(count, q75, r, max_views, math.radians(min_angle_deg)) for r in range(count.shape[0])]
+
+
+def depth_ranges(scene):
+    """depth_ranges (oracle/scene_oracle.c; neighbors.rs:162-189): per view (lo, hi) or None."""
+    from cumulus3d_amd.neighbors import SparseScene
+
+    s = scene if isinstance(scene, SparseScene) else SparseScene(scene)
+    lo = np.zeros(s.n, np.float64)
+    hi = np.zeros(s.n, np.float64)
+    has = np.zeros(s.n, np.uint8)
+    lib().pmo_depth_ranges(C.byref(s.c), lo.ctypes.data_as(C.POINTER(C.c_double)), hi.ctypes.data_as(C.POINTER(C.c_double)),
+                           has.ctypes.data_as(C.POINTER(C.c_uint8)))
+    return [(float(lo[v]), float(hi[v])) if has[v] else None for v in range(s.n)]
 
 
 def remove_speckles(depth, normal, cost, params):

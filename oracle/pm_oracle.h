@@ -86,6 +86,11 @@ int pmo_upsample(pmo_ctx* ctx, uint32_t level, const uint32_t* views, uint32_t n
                  const c3d_view_state* low, float sigma_s, float sigma_c, c3d_view_state* out);
 int pmo_median(pmo_ctx* ctx, uint32_t level, const uint32_t* views, uint32_t n,
                const c3d_view_state* in, c3d_view_state* out);
+/* scene preparation (scene_oracle.c; neighbors.rs:47-189); the struct is include/c3d_scene.h's */
+struct c3d_sparse_scene;
+int pmo_pair_stats(const struct c3d_sparse_scene* s, uint32_t* count, double* q75);
+int pmo_depth_ranges(const struct c3d_sparse_scene* s, double* dmin, double* dmax, uint8_t* has);
+
 /* read-out post-processing (post_oracle.c; postproc.rs:77-204) */
 int pmo_remove_speckles(float* depth, float* normal, const float* cost, uint32_t w, uint32_t h,
                         const c3d_post_params* p, uint64_t* removed);

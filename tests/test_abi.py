@@ -14,6 +14,8 @@ from cumulus3d_amd import abi, init_net, pm
 ROOT = os.path.dirname(os.path.dirname(os.path.abspath(__file__)))
 HEADER = os.path.join(ROOT, "include", "c3d_pm.h")
 INIT_HEADER = os.path.join(ROOT, "include", "c3d_init.h")
+FUSION_HEADER = os.path.join(ROOT, "include", "c3d_fusion.h")
+SCENE_HEADER = os.path.join(ROOT, "include", "c3d_scene.h")
 
 
 def declared_functions(header=HEADER, prefix="c3d_"):
@@ -43,6 +45,16 @@ def test_library_exports_the_initialiser():
     assert sorted(init_net.EXPORTS) == names
 
 
+def test_library_exports_fusion_and_scene_preparation():
+    from cumulus3d_amd import neighbors
+
+    L = C.CDLL(pm.library_path())
+    for header, exports in ((FUSION_HEADER, ["c3d_fuse_consistency"]), (SCENE_HEADER, neighbors.EXPORTS)):
+        names = declared_functions(header)
+        assert names and not [n for n in names if not hasattr(L, n)]
+        assert sorted(exports) == names
+
+
 STRUCTS = {
     "c3d_pm_options": abi.c3d_pm_options,
     "c3d_level_image": abi.c3d_level_image,
@@ -68,8 +80,21 @@ STRUCTS.update({
 })
 
 
+def _more_structs():
+    from cumulus3d_amd import fusion, neighbors
+
+    return {
+        "c3d_fusion_view": fusion.c3d_fusion_view,
+        "c3d_fusion_params": fusion.c3d_fusion_params,
+        "c3d_point_cloud": fusion.c3d_point_cloud,
+        "c3d_sparse_scene": neighbors.c3d_sparse_scene,
+    }
+
+
 def test_struct_layouts_match_the_header():
-    lines = ['#include <stdio.h>', '#include <stddef.h>', f'#include "{HEADER}"', f'#include "{INIT_HEADER}"', "int main(void) {"]
+    STRUCTS.update(_more_structs())
+    lines = ['#include <stdio.h>', '#include <stddef.h>', f'#include "{HEADER}"', f'#include "{INIT_HEADER}"',
+             f'#include "{FUSION_HEADER}"', f'#include "{SCENE_HEADER}"', "int main(void) {"]
     for name, cls in STRUCTS.items():
         lines.append(f'printf("{name} size %zu\\n", sizeof({name}));')
         for fname, _ in cls._fields_:

@@ -120,8 +120,13 @@ def _accuracy(sc, outs, views):
 
 @pytest.fixture(scope="module")
 def synth():
+    from cumulus3d_amd import neighbors
+
     sc = scenes.make_synth_scene()
-    return sc, scenes.select_sources(sc), scenes.depth_ranges(sc)
+    be = pm.HipPatchMatch()
+    src, ranges = neighbors.sources_and_ranges(be, sc)
+    be.close()
+    return sc, src, ranges
 
 
 @pytest.mark.parametrize("profile", ["fast", "quality"])

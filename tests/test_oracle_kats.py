@@ -121,8 +121,12 @@ def test_levels_and_pyramid_reference(oracle_lib):
 
 
 def test_depth_range_reference():
-    """neighbors.rs:193-198."""
-    a, b = scenes.depth_range_of([float(x) for x in range(1, 101)])
+    """neighbors.rs:193-198: the depths 1..100 give the range (1.5, 125)."""
+    import oracle
+
+    view = scenes.DenseView("v", 4, 4, np.array([10.0, 10.0, 1.5, 1.5]), np.eye(3), np.zeros(3), np.zeros((4, 4), np.uint8))
+    pts = [scenes.ScenePoint(np.array([0.0, 0.0, float(d)]), [0]) for d in range(100, 0, -1)]
+    (a, b), = oracle.depth_ranges(scenes.DenseScene([view], pts, None, None))
     assert abs(a - 1.5) < 1e-12 and abs(b - 125.0) < 1e-12
 
 

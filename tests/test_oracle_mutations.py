@@ -40,12 +40,13 @@ def mutants(tmp_path_factory):
     src = os.path.join(ROOT, "oracle", "pm_oracle.c")
     fsrc = os.path.join(ROOT, "oracle", "fusion_oracle.c")
     psrc = os.path.join(ROOT, "oracle", "post_oracle.c")
+    ssrc = os.path.join(ROOT, "oracle", "scene_oracle.c")
     paths = {}
     procs = []
     for m in CASES:
         out = str(d / f"libpm_oracle_mut{m}.so")
         cmd = ["gcc", "-O1", "-march=x86-64-v3", "-ffp-contract=off", "-fno-math-errno", "-fopenmp", "-fPIC", "-shared",
-               f"-DPMO_MUT={m}", "-o", out, src, fsrc, psrc, "-lm"]
+               f"-DPMO_MUT={m}", "-o", out, src, fsrc, psrc, ssrc, "-lm"]
         procs.append(subprocess.Popen(cmd))
         paths[m] = out
     for p in procs:

@@ -14,8 +14,8 @@ def test_oracle_meets_reference_ground_truth_bars():
     import oracle
 
     sc = scenes.make_synth_scene()
-    src = scenes.select_sources(sc)
-    ranges = scenes.depth_ranges(sc)
+    src = oracle.select_sources(sc)
+    ranges = oracle.depth_ranges(sc)
     h, w = sc.views[0].gray.shape
     nl = oracle.lib().pmo_num_levels(min(w, h), 3, 64)
     assert nl == 3
