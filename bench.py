@@ -145,10 +145,12 @@ def zone_main(args):
     be = pm.HipPatchMatch()
     net = init_net.InitNet(be, 8, 272, 480)
     net.set_params(np.load(wpath, allow_pickle=False).astype(np.float32))
+    # read-out post-processing of the fast profile (speckle removal + hole filling, densify.rs:455-462)
+    post = abi.PostParams.fast() if args.post else None
     plain_s = abi.Schedule(photometric_iters=args.photometric_iters, geometric_rounds=args.geometric_rounds,
-                           geometric_iters=args.geometric_iters)
+                           geometric_iters=args.geometric_iters, post=post)
     learned_s = init_net.LearnedSchedule(refine_iters=args.refine_iters, geometric_rounds=args.geometric_rounds,
-                                         geometric_iters=args.geometric_iters)
+                                         geometric_iters=args.geometric_iters, post=post)
     res = {}
     tune_rows = None
     with be.begin(inp) as s:
@@ -158,6 +160,7 @@ def zone_main(args):
             calib = views[:12]
             t = time.perf_counter()
             learned_s, tune_rows = autotune.tune_learned(s, net, calib, plain_s)
+            learned_s.post = post
             log(f"schedule autotuner on {len(calib)} views ({time.perf_counter() - t:.1f} s): "
                 + ", ".join(f"{c}: {a:.4f} in {ms:.0f} ms" for c, a, ms in tune_rows)
                 + f" -> refine {learned_s.refine_iters}, geometric {learned_s.geometric_rounds}x{learned_s.geometric_iters}, "
@@ -179,14 +182,17 @@ def zone_main(args):
             res[name] = (ms, outs, tms)
         # two plain runs that differ only in their random streams: the agreement a method can have
         reseeded = s.densify(views, abi.Schedule(photometric_iters=args.photometric_iters, geometric_rounds=args.geometric_rounds,
-                                                 geometric_iters=args.geometric_iters, first_run_id=101))
-    # accuracy of the learned maps against the plain ones (on the plain method's valid pixels)
+                                                 geometric_iters=args.geometric_iters, first_run_id=101, post=post))
+    # the plain method's read-out mask: raw depth > 0 and count >= max(1, min(min_num_consistent, nsrc))
+    # (densify.rs:446-454), before any post-processing
+    _, fl = abi.fast_profile()
+    valid = [(res["plain"][1][2][v] > 0) & (res["plain"][1][1][v] >= max(1, min(fl.min_num_consistent, len(src[v])))) for v in views]
+    # accuracy of the learned maps against the plain ones (raw depth and final normal on that mask)
     ok_d = ok_n = tot = 0
     for v in views:
-        # raw depth of both methods on the plain method's final (read-out) valid pixels
         dp, dl = res["plain"][1][2][v], res["learned"][1][2][v]
         npl, nle = res["plain"][1][0][v].normal, res["learned"][1][0][v].normal
-        m = res["plain"][1][0][v].depth > 0
+        m = valid[v]
         tot += int(m.sum())
         rel = np.abs(dl[m] - dp[m]) / dp[m]
         ok_d += int((rel <= 0.01).sum())
@@ -196,7 +202,7 @@ def zone_main(args):
     log(f"learned vs plain: depth within 1 % on {frac_d:.4f}, normal within 2 deg on {frac_n:.4f} of {tot} px")
     rd = rn = 0
     for v in views:
-        m = res["plain"][1][0][v].depth > 0
+        m = valid[v]
         dp, dr = res["plain"][1][2][v], reseeded[2][v]
         rd += int((np.abs(dr[m] - dp[m]) <= 0.01 * dp[m]).sum())
         cosang = np.clip((res["plain"][1][0][v].normal[m] * reseeded[0][v].normal[m]).sum(-1), -1, 1)
@@ -246,7 +252,12 @@ def zone_main(args):
                              "autotuned_on_views": 12 if tune_rows else 0,
                              "candidates": [[list(c), round(a, 5), round(ms, 1)] for c, a, ms in tune_rows] if tune_rows else None},
         "stage_ms_learned": {"init": round(res["learned"][2][-1].init_ms, 2), "photometric": round(res["learned"][2][-1].photometric_ms, 2),
-                             "geometric": round(res["learned"][2][-1].geometric_ms, 2), "filter": round(res["learned"][2][-1].filter_ms, 2)},
+                             "geometric": round(res["learned"][2][-1].geometric_ms, 2), "filter": round(res["learned"][2][-1].filter_ms, 2),
+                             "post": round(res["learned"][2][-1].post_ms, 2)},
+        "post_processing": {"speckles": bool(post), "fill_holes": bool(post),
+                            "plain_removed": int(res["plain"][2][-1].post_removed), "plain_filled": int(res["plain"][2][-1].post_filled),
+                            "learned_removed": int(res["learned"][2][-1].post_removed),
+                            "learned_filled": int(res["learned"][2][-1].post_filled)},
         "clouds": {"plain_points": len(clouds["plain"].positions), "learned_points": len(clouds["learned"].positions),
                    "median_distance_m": round(med, 5), "completeness_5cm": round(comp, 5)},
     }
@@ -270,6 +281,7 @@ def main():
     ap.add_argument("--graph", type=int, default=None, help="1 = replay the schedule as one hipGraph (default: batched48 only)")
     ap.add_argument("--refine-iters", type=int, default=2, help="zone108: refinement iterations after the learned init")
     ap.add_argument("--autotune", type=int, default=1, help="zone108: pick the learned schedule with the autotuner (1) or not (0)")
+    ap.add_argument("--post", type=int, default=1, help="zone108: the fast profile's speckle removal and hole filling (1) or none (0)")
     ap.add_argument("--cpu-rows", type=int, default=320)
     ap.add_argument("--no-cpu", action="store_true")
     ap.add_argument("--calibrate-copy", type=int, default=0, metavar="MIB",

@@ -7,6 +7,7 @@ is tuned separately by the engine (c3d_pm_autotune).
 """
 from __future__ import annotations
 
+import dataclasses
 import time
 from typing import List, Optional, Sequence, Tuple
 
@@ -34,7 +35,8 @@ def tune_learned(session: pm.Session, net: init_net.InitNet, views: Sequence[int
     """Returns (chosen LearnedSchedule, report rows [(candidate, agreement, ms)]). The first
     candidate (in cost order) whose agreement reaches `gate` wins; if none does, the one with the
     highest agreement."""
-    p_outs, _, p_raw, _ = session.densify(views, plain)
+    # the plain method's read-out mask before any post-processing
+    p_outs, _, p_raw, _ = session.densify(views, dataclasses.replace(plain, post=None))
     rows = []
     best = None
     for (ri, gr, gi, cm) in candidates:

@@ -271,6 +271,11 @@ __global__ __launch_bounds__(ROWS * 64) __attribute__((amdgpu_waves_per_eu(2, 2)
             ncand = 7;
         }
     }
+    // A candidate's cost is a sum of non-negative terms (weights > 0, costs in [0, 2], geometric
+    // errors in [0, gmax], lambda >= 0) divided by den: its partial sums only grow, so once
+    // nm / den reaches bcost the candidate cannot be accepted and its remaining sources are
+    // skipped. The accepted planes and costs are unchanged.
+    const bool early = P.lambda >= 0.f && (!GEO || P.gmax >= 0.f);
 #pragma nounroll
     for (int c = 0; c < ncand; ++c) {
         const float4 hc = cand[c * NT];
@@ -286,6 +291,7 @@ __global__ __launch_bounds__(ROWS * 64) __attribute__((amdgpu_waves_per_eu(2, 2)
                 float g;
                 float m = pair_cost_h<WR, WSTEP, GEO>(P, rw, ref, R, S, pr, lps[j], ifx, ify, x, y, hc, g);
                 nm += wj * (m + P.lambda * g);
+                if (early && nm / den >= bcost) break;
             }
             cc = nm / den;
         }
