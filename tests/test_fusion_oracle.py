@@ -70,3 +70,58 @@ def test_cloud_compare_closed_form():
     far = fusion.Cloud(pts + np.float32([0, 0, 0.1]) + 100, pts, a.colors, a.keys)
     med, comp = fusion.compare(a, far, 0.05)
     assert comp == 0.0
+
+
+def _two_down_views(hA, hB, w=64, h=48):
+    """Two cameras looking straight down at the plane z = 0 from heights hA, hB: every depth map is
+    the constant camera height, every normal (0, 0, -1) in camera coordinates."""
+    k = np.array([60.0, 60.0, 31.5, 23.5])
+    r = np.diag([1.0, -1.0, -1.0])
+    views = []
+    for c, g in (((0.0, 0.0, hA), 100), ((0.7, 0.3, hB), 200)):
+        c = np.array(c)
+        views.append(scenes.DenseView(f"v{g}", w, h, k, r, -r @ c, np.full((h, w), g, np.uint8)))
+    return scenes.DenseScene(views, [], None, None), k, r
+
+
+def test_inverse_variance_weights_follow_the_fourth_power_law():
+    """Fused point of a pixel of view 0 and its one agreeing pixel of view 1 against the closed
+    form of inverse-variance weighting: sigma_d = d^2 sigma_px / (f b) gives w ~ 1 / (d^4 sigma_px^2)
+    (baselines and focal lengths equal here), sigma_px = sigma0 + slope * clamp(cost, 0, 2); the
+    fused point and colour divide the two observations in the ratio w1 : w0."""
+    import oracle
+
+    w, h = 64, 48
+    for (hA, hB, cB, inv) in [(10.0, 10.0, 0.0, True), (10.0, 12.0, 0.0, True), (10.0, 10.0, 0.25, True), (10.0, 10.0, 5.0, True),
+                              (10.0, 12.0, 0.25, False)]:
+        sc, k, r = _two_down_views(hA, hB, w, h)
+        depth = [np.full((h, w), hA, np.float32), np.full((h, w), hB, np.float32)]
+        normal = [np.broadcast_to(np.array([0.0, 0.0, -1.0], np.float32), (h, w, 3)).copy() for _ in range(2)]
+        cost = [np.zeros((h, w), np.float32), np.full((h, w), cB, np.float32)]
+        keep = []
+        arr = fusion.fusion_views(sc, depth, normal, cost, [[1], [0]], [[1], [0]], keep)
+        p = fusion.FusionParams(min_consistent_views=1, consistency_num_images=1, mark_used=False, inverse_variance=inv)
+        cl = oracle.fuse_consistency(arr, p, 2 * w * h + 1)
+        sA, sB = 0.25, 0.25 + 1.0 * min(max(cB, 0.0), 2.0)
+        wA, wB = (1.0, 1.0) if not inv else (1.0 / (hA ** 4 * sA ** 2), 1.0 / (hB ** 4 * sB ** 2))
+        t = wB / (wA + wB)
+        cA, cBc = sc.views[0].center(), sc.views[1].center()
+        checked = 0
+        for (row, col) in [(5, 7), (23, 31), (40, 60)]:
+            ray = np.array([(col - k[2]) / k[0], (row - k[3]) / k[1], 1.0])
+            xA = cA + (r.T @ ray) * hA                        # the pixel's point on the plane
+            cam = r @ (xA - cBc)                              # in view 1
+            u, v = k[0] * cam[0] / cam[2] + k[2], k[1] * cam[1] / cam[2] + k[3]
+            qc, qr = np.floor(u + 0.5), np.floor(v + 0.5)
+            if not (0 <= qc < w and 0 <= qr < h):
+                continue
+            rq = np.array([(qc - k[2]) / k[0], (qr - k[3]) / k[1], 1.0])
+            xB = cBc + (r.T @ rq) * hB
+            want = xA + t * (xB - xA)
+            key = (row << 16) | col
+            i = np.nonzero(cl.keys == key)[0]
+            assert len(i) == 1
+            assert np.allclose(cl.positions[i[0]], want, atol=2e-5), (hA, hB, cB, inv)
+            assert abs(int(cl.colors[i[0], 0]) - (100 + 100 * t)) <= 0.5 + 1e-9
+            checked += 1
+        assert checked >= 2
