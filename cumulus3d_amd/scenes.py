@@ -219,6 +219,22 @@ def _round_half_away(x):
     return np.where(x >= 0, np.floor(x + 0.5), np.ceil(x - 0.5))
 
 
+def surface_distance(points: np.ndarray, cfg: SynthConfig = SynthConfig()) -> np.ndarray:
+    """Distance of world points (N, 3) to the floor-and-boxes surface (SynthScene::surface_distance,
+    synthetic.rs:296-310): the floor z = 0, or a box's nearest face from inside, its closest point
+    from outside."""
+    p = np.asarray(points, np.float64).reshape(-1, 3)
+    best = np.abs(p[:, 2])
+    for b in cfg.boxes:
+        lo, hi = np.asarray(b.min, np.float64), np.asarray(b.max, np.float64)
+        inside = np.all((p >= lo) & (p <= hi), axis=1)
+        d_in = np.minimum(p - lo, hi - p).min(axis=1)
+        q = np.maximum(np.maximum(lo - p, 0.0), p - hi)
+        d_out = np.sqrt((q * q).sum(axis=1))
+        best = np.minimum(best, np.where(inside, d_in, d_out))
+    return best
+
+
 def make_synth_scene(cfg: SynthConfig = SynthConfig()) -> DenseScene:
     """The reference's floor+boxes scene with ground truth (synthetic.rs:190-292)."""
     w, h, f = cfg.width, cfg.height, cfg.focal

@@ -150,3 +150,34 @@ def test_reference_ground_truth_bars(synth, profile):
     assert valid > 0.5
     for a, b, c, d in zip(raws, raws2, outs, again):
         assert _same(a, b) and _same(c.normal, d.normal)
+
+
+def test_reference_fusion_surface(synth):
+    """patchmatch_gpu.rs:119-132: the consistency fusion of the fast profile's depth maps (with its
+    speckle removal and hole filling) lies on the scene surface: more than 10,000 points, median
+    distance to the floor-and-boxes surface below 2 cm, fewer than 2 % farther than 10 cm."""
+    from cumulus3d_amd import fusion, neighbors
+
+    sc, src, ranges = synth
+    be = pm.HipPatchMatch()
+    h, w = sc.views[0].gray.shape
+    nl = pm.num_levels(min(w, h), 3, 64)
+    pmp, fl = abi.fast_profile()
+    inp = scenes.kernel_input(sc, src, ranges, pmp, fl, levels=[be.build_pyramid(v.gray, v.k, nl) for v in sc.views])
+    views = list(range(len(sc.views)))
+    with be.begin(inp) as s:
+        outs, _, _, tm = s.densify(views, abi.Schedule(post=abi.PostParams.fast()))
+    fp = fusion.FusionParams()
+    count, q75 = neighbors.pair_stats(be, sc)
+    overlap = [neighbors.select(count, q75, v, fp.consistency_num_images, 0.0) for v in views]
+    keep = []
+    arr = fusion.fusion_views(sc, [o.depth for o in outs], [o.normal for o in outs], [o.cost for o in outs], overlap, src, keep)
+    cloud = fusion.fuse(be, arr, fp, int(sum((o.depth > 0).sum() for o in outs)) + 1)
+    d = scenes.surface_distance(cloud.positions)
+    n = len(d)
+    print(f"{n} points, median surface distance {np.median(d):.4f} m, beyond 0.1 m {100.0 * (d > 0.1).mean():.3f} %, "
+          f"post removed {tm.post_removed} filled {tm.post_filled}")
+    assert n > 10_000
+    assert np.median(d) < 0.02
+    assert (d > 0.1).sum() * 50 < n
+    be.close()
