@@ -17,8 +17,9 @@ from . import abi, init_net, pm
 
 # (refine iterations, geometric rounds, geometric iterations, confidence below which the random
 # initialisation is used), cheapest first
-CANDIDATES: List[Tuple[int, int, int, float]] = [(1, 1, 1, 0.5), (1, 1, 1, 0.25), (1, 1, 1, 0.0), (2, 1, 1, 0.5), (2, 1, 1, 0.25),
-                                                 (2, 1, 2, 0.5), (3, 1, 2, 0.5), (4, 1, 2, 0.5), (5, 1, 2, 0.5)]
+CANDIDATES: List[Tuple[int, int, int, float]] = [(1, 0, 0, 0.5), (2, 0, 0, 0.5), (1, 1, 1, 0.5), (1, 1, 1, 0.25), (1, 1, 1, 0.0),
+                                                 (2, 1, 1, 0.5), (2, 1, 1, 0.25), (2, 1, 2, 0.5), (3, 1, 2, 0.5), (4, 1, 2, 0.5),
+                                                 (5, 1, 2, 0.5)]
 
 
 def depth_agreement(plain_outs, plain_raw, raw) -> float:
