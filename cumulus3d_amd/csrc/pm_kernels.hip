@@ -67,10 +67,15 @@ __device__ __forceinline__ float pair_cost_h(const KParams& P, const RefWinT<WR,
 
 // ================================================================ half step (patchmatch.cu:439-657)
 // GEO: geometric-consistency run (the geometric term is compiled in only there).
-template <int WR, int WSTEP, bool GEO, int ROWS>
+// TPP: lanes per pixel. With 2, the two lanes of a pixel (adjacent lanes) keep their own copies of
+// the pixel's state and split its NCC windows: the step-2 hypotheses by parity and the refinement
+// candidates by parity, exchanging the costs with a lane swap; every decision is then taken on both
+// lanes from the same values in the same order, so the results are those of one lane per pixel.
+template <int WR, int WSTEP, bool GEO, int ROWS, int TPP>
 __global__ __launch_bounds__(ROWS * 64) __attribute__((amdgpu_waves_per_eu(2, 2))) void k_half(KParams P) {
     constexpr int NT = ROWS * 64;
-    constexpr int TW = HALF_SPAN + 2 * WR;
+    constexpr int SPAN = HALF_SPAN / TPP;  // pixel span of one wave row (64 / TPP active pixels)
+    constexpr int TW = SPAN + 2 * WR;
     constexpr int TH = ROWS + 2 * WR;
     __shared__ float tile_s[WR > 0 ? TW * TH : 1];
     // per-thread hypothesis table: the 9 candidates of step 2, then the refinement candidates
@@ -78,13 +83,13 @@ __global__ __launch_bounds__(ROWS * 64) __attribute__((amdgpu_waves_per_eu(2, 2)
     // dynamic LDS: [reference tile (WR = 0 only)] [per-thread view weights]
     extern __shared__ float dyn_lds[];
     const int wr = WR > 0 ? WR : P.wr;
-    const int twd = HALF_SPAN + 2 * wr, thd = ROWS + 2 * wr;
+    const int twd = SPAN + 2 * wr, thd = ROWS + 2 * wr;
     float* tile = WR > 0 ? tile_s : dyn_lds;
     float* wv_lds = WR > 0 ? dyn_lds : dyn_lds + twd * thd;
 
     BatchItem it;
     int px0, py0;
-    if (!locate_tile(P, xcd_tile(blockIdx.x, P.ntiles), HALF_SPAN, ROWS, it, px0, py0)) return;
+    if (!locate_tile(P, xcd_tile(blockIdx.x, P.ntiles), SPAN, ROWS, it, px0, py0)) return;
     const LevelDesc R = P.lv[it.view * P.nlevels + P.level];
     const int w = it.w, h = it.h;
     {
@@ -98,9 +103,10 @@ __global__ __launch_bounds__(ROWS * 64) __attribute__((amdgpu_waves_per_eu(2, 2)
     }
     __syncthreads();
     const int lane = threadIdx.x & 63, row = threadIdx.x >> 6;
+    const int sub = lane & (TPP - 1);  // this lane's share of the pixel's windows
     const int y = py0 + row;
-    const int x = px0 + 2 * lane + ((y + P.color) & 1);
-    if (x >= w || y >= h) return;
+    const int x = px0 + 2 * (lane / TPP) + ((y + P.color) & 1);
+    if (x >= w || y >= h) return;  // both lanes of a pixel leave together
 
     const auto ref = tile_ref<WR, TW>(tile, px0 - wr, py0 - wr, twd);
     const ViewDesc vd = P.views[it.view];
@@ -181,8 +187,10 @@ __global__ __launch_bounds__(ROWS * 64) __attribute__((amdgpu_waves_per_eu(2, 2)
         wv[j * P.wv_stride] = 0.f;
         if (!(prior > 0.f)) continue;
         float m[9], gg[9];
+#pragma unroll
+        for (int q = 0; q < 9; ++q) m[q] = gg[q] = 0.f;
 #pragma nounroll
-        for (int i = 0; i < 9; ++i) {
+        for (int i = sub; i < 9; i += TPP) {
             float gi;
             const float mi = pair_cost_h<WR, WSTEP, GEO>(P, rw, ref, R, S, pr, lp, ifx, ify, x, y, hyp[i * NT], gi);
 #pragma unroll
@@ -191,6 +199,16 @@ __global__ __launch_bounds__(ROWS * 64) __attribute__((amdgpu_waves_per_eu(2, 2)
                     m[q] = mi;
                     gg[q] = gi;
                 }
+        }
+        if constexpr (TPP > 1) {
+#pragma unroll
+            for (int q = 0; q < 9; ++q) {
+                const float om = __shfl_xor(m[q], 1), og = __shfl_xor(gg[q], 1);
+                if ((q & 1) != sub) {
+                    m[q] = om;
+                    gg[q] = og;
+                }
+            }
         }
         int good = 0, bad = 0;
         float conf = 0.f;
@@ -275,11 +293,18 @@ __global__ __launch_bounds__(ROWS * 64) __attribute__((amdgpu_waves_per_eu(2, 2)
     // errors in [0, gmax], lambda >= 0) divided by den: its partial sums only grow, so once
     // nm / den reaches bcost the candidate cannot be accepted and its remaining sources are
     // skipped. The accepted planes and costs are unchanged.
+    // With two lanes per pixel each lane costs every other candidate against the step-2 best (a
+    // candidate that reaches it cannot be accepted later either, the best only decreasing), then
+    // both lanes accept in candidate order from the exchanged costs.
     const bool early = P.lambda >= 0.f && (!GEO || P.gmax >= 0.f);
+    float ccs[7];
+#pragma unroll
+    for (int q = 0; q < 7; ++q) ccs[q] = 3.f;  // above any cost: not accepted
 #pragma nounroll
-    for (int c = 0; c < ncand; ++c) {
+    for (int c = sub; c < ncand; c += TPP) {
         const float4 hc = cand[c * NT];
         float cc = 2.f;
+        bool out = false;
         if (den > 0.f) {
             float nm = 0.f;
 #pragma nounroll
@@ -291,18 +316,42 @@ __global__ __launch_bounds__(ROWS * 64) __attribute__((amdgpu_waves_per_eu(2, 2)
                 float g;
                 float m = pair_cost_h<WR, WSTEP, GEO>(P, rw, ref, R, S, pr, lps[j], ifx, ify, x, y, hc, g);
                 nm += wj * (m + P.lambda * g);
-                if (early && nm / den >= bcost) break;
+                if (early && nm / den >= bcost) {
+                    out = true;
+                    break;
+                }
             }
             cc = nm / den;
         }
-        if (cc < bcost) {
-            bcost = cc;
-            best = hc;
+        if constexpr (TPP == 1) {
+            if (cc < bcost) {
+                bcost = cc;
+                best = hc;
+            }
+        } else {
+#pragma unroll
+            for (int q = 0; q < 7; ++q)
+                if (q == c) ccs[q] = out ? 3.f : cc;
         }
     }
-    pl[pix] = best;
-    P.cost[base + pix] = bcost;
-    P.bview[base + pix] = (uint8_t)vbest;
+    if constexpr (TPP > 1) {
+#pragma unroll
+        for (int q = 0; q < 7; ++q) {
+            const float o = __shfl_xor(ccs[q], 1);
+            if ((q & 1) != sub) ccs[q] = o;
+        }
+#pragma unroll
+        for (int q = 0; q < 7; ++q)
+            if (q < ncand && ccs[q] < bcost) {
+                bcost = ccs[q];
+                best = cand[q * NT];
+            }
+    }
+    if (sub == 0) {
+        pl[pix] = best;
+        P.cost[base + pix] = bcost;
+        P.bview[base + pix] = (uint8_t)vbest;
+    }
 }
 
 // Stages the reference rows of a 64 x 4 pixel tile (plus the window margin) in LDS.
@@ -650,9 +699,12 @@ __global__ void k_pyramid_step(const uint8_t* src, int w, int h, uint8_t* dst, f
 
 // ================================================================ launchers
 
-int half_tiles(const BatchItem* items, int n, int rows) {
+bool half_shape_ok(int rows, int tpp) { return (rows == 2 && tpp == 1) || (rows == 4 && (tpp == 1 || tpp == 2)); }
+
+int half_tiles(const BatchItem* items, int n, int rows, int tpp) {
+    const int span = HALF_SPAN / tpp;
     int t = 0;
-    for (int i = 0; i < n; ++i) t += ((items[i].w + HALF_SPAN - 1) / HALF_SPAN) * ((items[i].h + rows - 1) / rows);
+    for (int i = 0; i < n; ++i) t += ((items[i].w + span - 1) / span) * ((items[i].h + rows - 1) / rows);
     return t;
 }
 int pix_tiles(const BatchItem* items, int n) {
@@ -662,24 +714,27 @@ int pix_tiles(const BatchItem* items, int n) {
 }
 
 // dynamic LDS bytes of the any-radius (WR = 0) reference tiles
-static size_t half_tile_bytes(int wr, int rows) { return (size_t)(HALF_SPAN + 2 * wr) * (rows + 2 * wr) * sizeof(float); }
+static size_t half_tile_bytes(int wr, int rows, int tpp) { return (size_t)(HALF_SPAN / tpp + 2 * wr) * (rows + 2 * wr) * sizeof(float); }
 static size_t pix_tile_bytes(int wr) { return (size_t)(PIX_TW + 2 * wr) * (PIX_TH + 2 * wr) * sizeof(float); }
 
-template <int WR, int WSTEP, int ROWS>
+template <int WR, int WSTEP, int ROWS, int TPP>
 static void launch_half_rows(const KParams& P, size_t lds, hipStream_t s) {
-    if (P.geometric) hipLaunchKernelGGL((k_half<WR, WSTEP, true, ROWS>), dim3(P.ntiles), dim3(ROWS * 64), lds, s, P);
-    else hipLaunchKernelGGL((k_half<WR, WSTEP, false, ROWS>), dim3(P.ntiles), dim3(ROWS * 64), lds, s, P);
+    if (P.geometric) hipLaunchKernelGGL((k_half<WR, WSTEP, true, ROWS, TPP>), dim3(P.ntiles), dim3(ROWS * 64), lds, s, P);
+    else hipLaunchKernelGGL((k_half<WR, WSTEP, false, ROWS, TPP>), dim3(P.ntiles), dim3(ROWS * 64), lds, s, P);
 }
 
-// P.ntiles must count tiles of `rows` rows (half_tiles(..., rows)); rows is 2 or 4
+// P.ntiles must count the tiles of the shape (half_tiles(..., rows, tpp)); the compiled shapes are
+// (rows, lanes per pixel) = (2, 1), (4, 1) and (4, 2)
 template <int WR, int WSTEP>
-static hipError_t launch_half(const KParams& P0, int rows, hipStream_t s) {
+static hipError_t launch_half(const KParams& P0, int rows, int tpp, hipStream_t s) {
+    if (!half_shape_ok(rows, tpp)) return hipErrorInvalidValue;
     KParams P = P0;
     P.wv_stride = rows * 64;
     size_t lds = (size_t)P.wv_stride * (P.max_nsrc > 0 ? P.max_nsrc : 1) * sizeof(float);
-    if (WR == 0) lds += half_tile_bytes(P.wr, rows);
-    if (rows == 2) launch_half_rows<WR, WSTEP, 2>(P, lds, s);
-    else launch_half_rows<WR, WSTEP, 4>(P, lds, s);
+    if (WR == 0) lds += half_tile_bytes(P.wr, rows, tpp);
+    if (rows == 2) launch_half_rows<WR, WSTEP, 2, 1>(P, lds, s);
+    else if (tpp == 1) launch_half_rows<WR, WSTEP, 4, 1>(P, lds, s);
+    else launch_half_rows<WR, WSTEP, 4, 2>(P, lds, s);
     return hipGetLastError();
 }
 template <int WR, int WSTEP>
@@ -700,7 +755,7 @@ static hipError_t launch_pair_costs(const KParams& P, int view, int slot, int n,
 }
 template <int WR, int WSTEP>
 static const void* half_symbol() {
-    return reinterpret_cast<const void*>(&k_half<WR, WSTEP, false, 4>);
+    return reinterpret_cast<const void*>(&k_half<WR, WSTEP, false, 4, 1>);
 }
 
 template <int WR, int WSTEP>

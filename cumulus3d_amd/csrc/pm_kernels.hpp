@@ -8,7 +8,7 @@ namespace c3d {
 // One compiled window variant (radius, step) of the window-templated kernels.
 struct KernelSet {
     int wr, wstep;
-    hipError_t (*half)(const KParams&, int rows, hipStream_t);
+    hipError_t (*half)(const KParams&, int rows, int tpp, hipStream_t);
     hipError_t (*eval)(const KParams&, int mode, hipStream_t);
     hipError_t (*filter)(const KParams&, hipStream_t);
     hipError_t (*pair_costs)(const KParams&, int view, int slot, int n, const int* xy, const float4* planes, float* out,
@@ -21,8 +21,11 @@ struct KernelSet {
 // outside those bounds.
 const KernelSet* kernel_set(int wr, int wstep, bool any_radius);
 
+// half-step workgroup shapes compiled: (rows of a workgroup, lanes per pixel) = (2, 1), (4, 1), (4, 2)
+bool half_shape_ok(int rows, int tpp);
+
 // workgroup counts of the two tilings for a batch
-int half_tiles(const BatchItem* items, int n, int rows);
+int half_tiles(const BatchItem* items, int n, int rows, int tpp);
 int pix_tiles(const BatchItem* items, int n);
 
 hipError_t launch_upsample(const KParams& P, hipStream_t s);

@@ -174,8 +174,10 @@ struct c3d_pm_session {
     std::vector<BatchItem> h_batch_lv;
     std::vector<long long> h_snap_lv;
     std::vector<uint8_t> h_need;
-    // rows of active pixels per half-step workgroup (options.tile_rows, or c3d_pm_autotune)
+    // half-step workgroup shape: rows of active pixels and lanes per pixel (options.tile_rows and
+    // options.threads_per_pixel, or c3d_pm_autotune)
     int half_rows = 4;
+    int half_tpp = 1;
     // the captured schedule (options.use_graph) and the key it was captured for
     hipGraphExec_t gexec = nullptr;
     std::vector<uint8_t> gkey;
@@ -356,7 +358,7 @@ int launch_run(c3d_pm_session* s, KParams k, const std::vector<BatchItem>& items
     hipStream_t st = s->be->stream;
     k.ntiles = pix_tiles(items.data(), (int)items.size());
     HIP_TRY(s->ks->eval(k, 1, st));
-    k.ntiles = half_tiles(items.data(), (int)items.size(), s->half_rows);
+    k.ntiles = half_tiles(items.data(), (int)items.size(), s->half_rows, s->half_tpp);
     for (uint32_t t = 0; t < iterations; ++t) {
         iteration_constants(pm, geometric, t, k);
         for (int color = 0; color < 2; ++color) {
@@ -369,7 +371,7 @@ int launch_run(c3d_pm_session* s, KParams k, const std::vector<BatchItem>& items
                 }
                 HIP_TRY(hipEventRecord(s->pool[s->pool_used], st));
             }
-            HIP_TRY(s->ks->half(k, s->half_rows, st));
+            HIP_TRY(s->ks->half(k, s->half_rows, s->half_tpp, st));
             if (timed) {
                 HIP_TRY(hipEventRecord(s->pool[s->pool_used + 1], st));
                 s->pool_used += 2;
@@ -443,6 +445,14 @@ int c3d_pm_create(int ordinal, const c3d_pm_options* opts, c3d_pm_backend** out)
     be->device = ordinal;
     be->name = std::string("hip(") + (pr.name[0] ? pr.name : "AMD Instinct") + ", " + pr.gcnArchName + ")";
     if (opts) be->opts = *opts;
+    {
+        const int rows = be->opts.tile_rows, tpp = be->opts.threads_per_pixel;
+        if ((rows != 0 && rows != 2 && rows != 4) || (tpp != 0 && tpp != 1 && tpp != 2) ||
+            !half_shape_ok(rows == 2 ? 2 : 4, tpp == 2 ? 2 : 1)) {
+            delete be;
+            return fail(C3D_INVALID_ARGUMENT, "half-step shapes are (tile_rows, threads_per_pixel) = (2, 1), (4, 1), (4, 2)");
+        }
+    }
     if (be->opts.batch_pixels <= 0) be->opts.batch_pixels = 1LL << 40;
     if (be->opts.max_batch <= 0) be->opts.max_batch = 64;
     if (hipSetDevice(ordinal) != hipSuccess || hipStreamCreateWithFlags(&be->stream, hipStreamNonBlocking) != hipSuccess) {
@@ -607,6 +617,7 @@ int c3d_pm_begin(c3d_pm_backend* be, const c3d_kernel_input* in, c3d_pm_session*
     s->pm_copy = pm;
     s->filter_copy = fl;
     s->half_rows = be->opts.tile_rows == 2 ? 2 : 4;
+    s->half_tpp = be->opts.threads_per_pixel == 2 ? 2 : 1;
     *out = s;
     return C3D_OK;
 }
@@ -1049,6 +1060,7 @@ std::vector<uint8_t> graph_key(c3d_pm_session* s, const uint32_t* views, uint32_
     put(views, n * sizeof(uint32_t));
     put(sch, sizeof *sch);
     put(&s->half_rows, sizeof s->half_rows);
+    put(&s->half_tpp, sizeof s->half_tpp);
     const void* ptrs[] = {s->plane.p, s->cost.p, s->bview.p, s->count.p, s->snap.p, s->plane2.p, s->need.p, s->low.p,
                           s->up.p, s->cinit.p, s->prior.p, s->d_batch_lv.p, s->d_snap_lv.p};
     put(ptrs, sizeof ptrs);
@@ -1397,7 +1409,8 @@ int c3d_pm_densify_learned(c3d_pm_session* s, c3d_init_net* net, const uint32_t*
     return C3D_OK;
 }
 
-int c3d_pm_autotune(c3d_pm_session* s, const uint32_t* views, uint32_t n, uint32_t level, uint32_t* rows_out, double* ms_out) {
+int c3d_pm_autotune(c3d_pm_session* s, const uint32_t* views, uint32_t n, uint32_t level, uint32_t* rows_out, uint32_t* tpp_out,
+                    double* ms_out) {
     if (!s || !views || n == 0) return fail(C3D_INVALID_ARGUMENT, "null argument");
     int rc = set_device(s->be);
     if (rc) return rc;
@@ -1409,24 +1422,25 @@ int c3d_pm_autotune(c3d_pm_session* s, const uint32_t* views, uint32_t n, uint32
     int max_nsrc;
     rc = set_batch(s, views, 0, n, level, items, npix, max_nsrc);
     if (rc) return rc;
-    // scratch states: a random-init evaluation, then two timed iterations per tile height after one untimed
+    // scratch states: a random-init evaluation, then two timed iterations per shape after one untimed
     KParams k = params_for(s, level, false, items, max_nsrc);
     k.run_id = 0x7FFFFFFF;
     k.random_init = 1;
     k.ntiles = pix_tiles(items.data(), (int)items.size());
     HIP_TRY(s->ks->eval(k, 1, st));
-    const int cand[2] = {2, 4};
-    double ms[2] = {0.0, 0.0};
-    for (int c = 0; c < 2; ++c) {
+    const int cand[3][2] = {{2, 1}, {4, 1}, {4, 2}};
+    double ms[3] = {0.0, 0.0, 0.0};
+    int bestc = 0;
+    for (int c = 0; c < 3; ++c) {
         KParams h = k;
         h.random_init = 0;
-        h.ntiles = half_tiles(items.data(), (int)items.size(), cand[c]);
+        h.ntiles = half_tiles(items.data(), (int)items.size(), cand[c][0], cand[c][1]);
         for (int it = 0; it < 3; ++it) {
             iteration_constants(s->pm_copy, false, (uint32_t)it, h);
             if (it == 1) HIP_TRY(hipEventRecord(s->ev[0], st));
             for (int color = 0; color < 2; ++color) {
                 h.color = color;
-                HIP_TRY(s->ks->half(h, cand[c], st));
+                HIP_TRY(s->ks->half(h, cand[c][0], cand[c][1], st));
             }
         }
         HIP_TRY(hipEventRecord(s->ev[1], st));
@@ -1434,13 +1448,14 @@ int c3d_pm_autotune(c3d_pm_session* s, const uint32_t* views, uint32_t n, uint32
         float t = 0.f;
         HIP_TRY(hipEventElapsedTime(&t, s->ev[0], s->ev[1]));
         ms[c] = t;
+        if (ms[c] < ms[bestc]) bestc = c;
     }
-    s->half_rows = ms[0] < ms[1] ? cand[0] : cand[1];
+    s->half_rows = cand[bestc][0];
+    s->half_tpp = cand[bestc][1];
     if (rows_out) *rows_out = (uint32_t)s->half_rows;
-    if (ms_out) {
-        ms_out[0] = ms[0];
-        ms_out[1] = ms[1];
-    }
+    if (tpp_out) *tpp_out = (uint32_t)s->half_tpp;
+    if (ms_out)
+        for (int c = 0; c < 3; ++c) ms_out[c] = ms[c];
     return C3D_OK;
 }
 

@@ -78,7 +78,7 @@ def load() -> C.CDLL:
                                      C.POINTER(abi.c3d_timing)]
         L.c3d_pm_pair_costs.argtypes = [C.c_void_p, C.c_uint32, C.c_uint32, C.c_uint32, C.c_uint32,
                                         C.POINTER(C.c_int32), C.POINTER(C.c_float), C.POINTER(C.c_float)]
-        L.c3d_pm_autotune.argtypes = [C.c_void_p, u32p, C.c_uint32, C.c_uint32, u32p, C.POINTER(C.c_double)]
+        L.c3d_pm_autotune.argtypes = [C.c_void_p, u32p, C.c_uint32, C.c_uint32, u32p, u32p, C.POINTER(C.c_double)]
         L.c3d_pm_autotune.restype = C.c_int
         L.c3d_pm_num_levels.argtypes = [C.c_uint32, C.c_uint32, C.c_uint32]
         L.c3d_pm_num_levels.restype = C.c_uint32
@@ -292,11 +292,12 @@ class Session:
         return outs, counts, raws, tm
 
     def autotune(self, views: Sequence[int], level: int = 0):
-        """c3d_pm_autotune: the tile height kept for this session and the timings (2 rows, 4 rows)."""
-        rows = C.c_uint32()
-        ms = (C.c_double * 2)()
-        _check(self.L, self.L.c3d_pm_autotune(self.h, abi.views_array(views), len(views), level, C.byref(rows), ms))
-        return int(rows.value), (ms[0], ms[1])
+        """c3d_pm_autotune: the half-step shape kept for this session, (rows, lanes per pixel), and
+        the timings of (2, 1), (4, 1), (4, 2)."""
+        rows, tpp = C.c_uint32(), C.c_uint32()
+        ms = (C.c_double * 3)()
+        _check(self.L, self.L.c3d_pm_autotune(self.h, abi.views_array(views), len(views), level, C.byref(rows), C.byref(tpp), ms))
+        return (int(rows.value), int(tpp.value)), (ms[0], ms[1], ms[2])
 
     def pair_costs(self, level: int, view: int, slot: int, xy: np.ndarray, planes: np.ndarray) -> np.ndarray:
         """Per-hypothesis photometric cost of explicit planes (c3d_pm_pair_costs)."""

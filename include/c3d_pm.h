@@ -56,7 +56,8 @@ typedef struct c3d_pm_options {
     int64_t batch_pixels;     /* pixel budget of one launch [2,200,000] (batch_pixels) */
     int32_t max_batch;        /* views per launch [64] (max_batch) */
     int32_t tile_rows;        /* rows of active pixels per half-step workgroup (2 or 4); 0 = 4 until c3d_pm_autotune */
-    int32_t threads_per_pixel;/* lanes cooperating on one pixel (1, 2 or 4); 0 = tuned default */
+    int32_t threads_per_pixel;/* lanes per pixel in the half step: 1, or 2 with tile_rows 4 (the two lanes
+                                 split the pixel's NCC windows); 0 = 1 until c3d_pm_autotune */
     int32_t use_graph;        /* 1 = capture each run's iteration loop in a hipGraph; 0 = plain launches */
     int32_t any_radius;       /* 1 = always the any-radius kernels (weights recomputed per sample), even
                                  for the windows with a register-resident build; 0 = fastest available */
@@ -343,13 +344,14 @@ int c3d_fill_holes(c3d_pm_backend* backend, float* depth, float* normal, float* 
 
 /* ---------------------------------------------------------------- autotuner (no reference counterpart) */
 
-/* Tile autotuner: times the red-black half step with each compiled tile height (2 and 4 rows of 64
- * active pixels per workgroup) on `views` at `level` (a random initialisation and two timed
- * photometric iterations on the session's scratch states, after one untimed) and keeps the faster
- * for the session's later runs. rows_out (optional) receives the choice, ms_out[2] (optional) the
- * times of the two iterations for 2 and 4 rows. */
+/* Tile autotuner: times the red-black half step with each compiled workgroup shape — (rows of
+ * active pixels, lanes per pixel) = (2, 1), (4, 1), (4, 2); 64 / lanes active pixels per wave row —
+ * on `views` at `level` (a random initialisation and two timed photometric iterations on the
+ * session's scratch states, after one untimed) and keeps the fastest for the session's later runs.
+ * Every shape gives identical results. rows_out / tpp_out (optional) receive the choice, ms_out[3]
+ * (optional) the times of the two iterations per shape in that order. */
 int c3d_pm_autotune(c3d_pm_session* session, const uint32_t* views, uint32_t num_views, uint32_t level,
-                    uint32_t* rows_out, double* ms_out);
+                    uint32_t* rows_out, uint32_t* tpp_out, double* ms_out);
 
 /* ---------------------------------------------------------------- scene preparation */
 

@@ -114,16 +114,27 @@ def test_quality_window_costs_match_fp64():
 
 
 def test_tile_rows_identical_and_autotune():
-    """Half-step workgroups of 2 and 4 rows give identical states; the autotuner keeps one of them."""
+    """Half-step workgroups of 2 and 4 rows, and 4 rows with two lanes per pixel (register and
+    any-radius windows), give identical states and read-out counts (photometric and geometric
+    runs); the autotuner keeps one of the shapes."""
     sc, inp = _input(192, 108, 5, 2)
     outs = []
-    for rows in (2, 4):
+    for rows, tpp, anyr in ((2, 1, 0), (4, 1, 0), (4, 2, 0), (4, 2, 1)):
         opts = abi.c3d_pm_options()
         opts.tile_rows = rows
+        opts.threads_per_pixel = tpp
+        opts.any_radius = anyr  # the any-radius kernels with two lanes per pixel too
         outs.append(_run(pm.HipPatchMatch(0, opts), inp))
-    for x, y in zip(outs[0][0], outs[1][0]):
-        assert _same(x.depth, y.depth) and _same(x.normal, y.normal) and _same(x.cost, y.cost)
+    for other in outs[1:]:
+        for x, y in zip(outs[0][0], other[0]):
+            assert _same(x.depth, y.depth) and _same(x.normal, y.normal) and _same(x.cost, y.cost)
+        for a, b in zip(outs[0][1], other[1]):
+            assert np.array_equal(a, b)
     with pm.HipPatchMatch().begin(inp) as s:
-        rows, ms = s.autotune([0, 1, 2])
-    print(f"autotune: rows {rows}, ms {ms}")
-    assert rows in (2, 4) and min(ms) > 0
+        shape, ms = s.autotune([0, 1, 2])
+    print(f"autotune: (rows, lanes per pixel) {shape}, ms {ms}")
+    assert shape in ((2, 1), (4, 1), (4, 2)) and min(ms) > 0
+    bad = abi.c3d_pm_options()
+    bad.tile_rows, bad.threads_per_pixel = 2, 2
+    with pytest.raises(pm.PmError):
+        pm.HipPatchMatch(0, bad)
