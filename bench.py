@@ -291,6 +291,8 @@ def main():
     ap.add_argument("--graph", type=int, default=None, help="1 = replay the schedule as one hipGraph (default: batched48 only)")
     ap.add_argument("--refine-iters", type=int, default=2, help="zone108: refinement iterations after the learned init")
     ap.add_argument("--autotune", type=int, default=1, help="zone108: pick the learned schedule with the autotuner (1) or not (0)")
+    ap.add_argument("--tile-autotune", type=int, default=1,
+                    help="zone7/batched48: pick the half step's workgroup shape with c3d_pm_autotune before timing (1) or keep (4, 1) (0)")
     ap.add_argument("--post", type=int, default=1, help="zone108: the fast profile's speckle removal and hole filling (1) or none (0)")
     ap.add_argument("--cpu-rows", type=int, default=320)
     ap.add_argument("--no-cpu", action="store_true")
@@ -330,6 +332,12 @@ def main():
     be = pm.HipPatchMatch(0, opts)
     sess = be.begin(inp)
     log(f"session open on {be.name()}, graph {use_graph}")
+    shape = (4, 1)
+    if args.tile_autotune:
+        # the half step's workgroup shape (rows, lanes per pixel), outside the timed region
+        shape, shape_ms = sess.autotune(views)
+        log(f"tile autotuner: (rows, lanes per pixel) {shape}; two iterations (2,1) {shape_ms[0]:.1f} ms, "
+            f"(4,1) {shape_ms[1]:.1f} ms, (4,2) {shape_ms[2]:.1f} ms")
     for i in range(args.warmup):
         sess.densify(views, sched, download=False)
         log(f"warmup {i + 1}/{args.warmup} done")
@@ -351,7 +359,9 @@ def main():
         # half-step launch times and the stage split
         sess.close()
         be.close()
-        be = pm.HipPatchMatch(0)
+        eopts = abi.c3d_pm_options()
+        eopts.tile_rows, eopts.threads_per_pixel = shape
+        be = pm.HipPatchMatch(0, eopts)
         sess = be.begin(inp)
         _, _, _, tm = sess.densify(views, sched, download=False)
         tms = [tm]
@@ -402,6 +412,7 @@ def main():
             "geometric_rounds": args.geometric_rounds,
             "geometric_iters": args.geometric_iters,
             "graph": use_graph,
+            "tile": {"rows": shape[0], "lanes_per_pixel": shape[1], "autotuned": bool(args.tile_autotune)},
         },
         "roofline": {
             "bound": "hbm",
