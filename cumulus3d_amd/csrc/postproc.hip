@@ -9,7 +9,8 @@
 // connected components of the 4-neighbour graph of joining pairs, whatever the visiting order,
 // and erasing one component never changes another. Here they are found by concurrent union-find
 // (atomicMin links, every label points to a smaller index of its component: the root is the
-// component's first pixel, as in the reference), counted with wave-aggregated atomics, and the
+// component's first pixel, as in the reference), first inside 16 x 16 tiles in LDS, then across
+// the tile edges in global memory, counted with wave-aggregated atomics, and the
 // median cost of every component below the area bound comes from one radix sort of
 // (root, cost in total order) keys: the key at position start + size/2 of a root's run is the
 // element costs[len / 2] of the reference's sorted list.
@@ -108,27 +109,100 @@ __global__ void k_clear(unsigned long long* c, int n) {
     for (int j = (int)threadIdx.x; j < n; j += (int)blockDim.x) c[j] = 0ull;
 }
 
-__global__ __launch_bounds__(256) void k_cc_init(const PItem* items, const float4* plane, unsigned* label, unsigned* size) {
-    PItem it;
-    long long i;
-    int x, y;
-    if (!item_pixel(items, it, i, x, y)) return;
-    label[i] = !(plane[i].w <= 0.0f) ? (unsigned)i : NONE;  // postproc.rs:90: depth <= 0 is no pixel
-    size[i] = 0u;
+// Components inside 16 x 16 tiles first, with the labels and the union-find in LDS; each tile's
+// labels then point to the tile-local root, which is the smallest global index of its piece (the
+// tile's row-major order is the image's). Only the pairs across tile edges are united in global
+// memory (k_cc_border).
+constexpr int CC_T = 16;
+
+__device__ __forceinline__ unsigned find_lds(const unsigned* L, unsigned a) {
+    unsigned p = L[a];
+    while (p != a) {
+        a = p;
+        p = __hip_atomic_load(L + a, __ATOMIC_RELAXED, __HIP_MEMORY_SCOPE_WORKGROUP);
+    }
+    return a;
 }
 
-__global__ __launch_bounds__(256) void k_cc_merge(const PItem* items, const float4* plane, unsigned* label, float rel) {
-    PItem it;
-    long long i;
-    int x, y;
-    if (!item_pixel(items, it, i, x, y)) return;
+__device__ void unite_lds(unsigned* L, unsigned a, unsigned b) {
+    for (;;) {
+        a = find_lds(L, a);
+        b = find_lds(L, b);
+        if (a == b) return;
+        if (a < b) {
+            const unsigned old = atomicMin(L + b, a);
+            if (old == b) return;
+            b = old;
+        } else {
+            const unsigned old = atomicMin(L + a, b);
+            if (old == a) return;
+            a = old;
+        }
+    }
+}
+
+__device__ __forceinline__ bool cc_tile(const PItem& it, int& x0, int& y0) {
+    const int tx = (it.w + CC_T - 1) / CC_T, ty = (it.h + CC_T - 1) / CC_T;
+    if ((long long)blockIdx.x >= (long long)tx * ty) return false;
+    x0 = (int)(blockIdx.x % tx) * CC_T;
+    y0 = (int)(blockIdx.x / tx) * CC_T;
+    return true;
+}
+
+__global__ __launch_bounds__(256) void k_cc_local(const PItem* items, const float4* plane, unsigned* label, unsigned* size, float rel) {
+    __shared__ unsigned L[CC_T * CC_T];
+    __shared__ float D[CC_T * CC_T];
+    const PItem it = items[blockIdx.y];
+    int x0, y0;
+    if (!cc_tile(it, x0, y0)) return;  // uniform over the workgroup
+    const int t = threadIdx.x, lx = t % CC_T, ly = t / CC_T;
+    const int x = x0 + lx, y = y0 + ly;
+    const bool in = x < it.w && y < it.h;
+    const long long i = it.off + (long long)y * it.w + x;
+    const float d = in ? plane[i].w : 0.0f;
+    const bool active = in && !(d <= 0.0f);  // postproc.rs:90: depth <= 0 is no pixel
+    D[t] = active ? d : 0.0f;
+    L[t] = active ? (unsigned)t : NONE;
+    if (in) size[i] = 0u;
+    __syncthreads();
+    if (active) {
+        if (lx + 1 < CC_T && x + 1 < it.w) {
+            const float dj = D[t + 1];
+            if (!(dj <= 0.0f) && joins(d, dj, rel)) unite_lds(L, (unsigned)t, (unsigned)(t + 1));
+        }
+        if (ly + 1 < CC_T && y + 1 < it.h) {
+            const float dj = D[t + CC_T];
+            if (!(dj <= 0.0f) && joins(d, dj, rel)) unite_lds(L, (unsigned)t, (unsigned)(t + CC_T));
+        }
+    }
+    __syncthreads();
+    if (in) {
+        unsigned g = NONE;
+        if (active) {
+            const unsigned r = find_lds(L, (unsigned)t);
+            g = (unsigned)(it.off + (long long)(y0 + (int)(r / CC_T)) * it.w + x0 + (int)(r % CC_T));
+        }
+        label[i] = g;
+    }
+}
+
+// unions across the right and bottom edges of every tile
+__global__ __launch_bounds__(256) void k_cc_border(const PItem* items, const float4* plane, unsigned* label, float rel) {
+    const PItem it = items[blockIdx.y];
+    int x0, y0;
+    if (!cc_tile(it, x0, y0)) return;
+    const int t = threadIdx.x, lx = t % CC_T, ly = t / CC_T;
+    if (lx != CC_T - 1 && ly != CC_T - 1) return;
+    const int x = x0 + lx, y = y0 + ly;
+    if (x >= it.w || y >= it.h) return;
+    const long long i = it.off + (long long)y * it.w + x;
     const float di = plane[i].w;
     if (di <= 0.0f) return;
-    if (x + 1 < it.w) {
+    if (lx == CC_T - 1 && x + 1 < it.w) {
         const float dj = plane[i + 1].w;
         if (!(dj <= 0.0f) && joins(di, dj, rel)) unite(label, (unsigned)i, (unsigned)(i + 1));
     }
-    if (y + 1 < it.h) {
+    if (ly == CC_T - 1 && y + 1 < it.h) {
         const float dj = plane[i + it.w].w;
         if (!(dj <= 0.0f) && joins(di, dj, rel)) unite(label, (unsigned)i, (unsigned)(i + it.w));
     }
@@ -374,6 +448,9 @@ int post_run(PostWork* w, const PostView* views, int n, float4* plane, float* co
     unsigned long long* d_removed = w->counters + 1;
     unsigned long long* d_filled = w->counters + 1 + n;
     const dim3 grid((unsigned)((maxpix + 255) / 256), (unsigned)n);
+    long long maxtiles = 0;
+    for (const PItem& q : it) maxtiles = std::max(maxtiles, (long long)((q.w + CC_T - 1) / CC_T) * ((q.h + CC_T - 1) / CC_T));
+    const dim3 cc_grid((unsigned)maxtiles, (unsigned)n);
     bool any_small = false;
     for (const PItem& q : it) any_small = any_small || q.min_area > 1u;
     if (p.remove_speckles && any_small) {
@@ -381,8 +458,8 @@ int post_run(PostWork* w, const PostView* views, int n, float4* plane, float* co
         P_TRY(grow(w->size, w->n_size, (size_t)tot));
         P_TRY(grow(w->keys, w->n_keys, (size_t)tot));
         P_TRY(grow(w->erase, w->n_erase, (size_t)tot));
-        hipLaunchKernelGGL(k_cc_init, grid, dim3(256), 0, st, w->items, plane, w->label, w->size);
-        hipLaunchKernelGGL(k_cc_merge, grid, dim3(256), 0, st, w->items, plane, w->label, p.speckle_rel_depth);
+        hipLaunchKernelGGL(k_cc_local, cc_grid, dim3(256), 0, st, w->items, plane, w->label, w->size, p.speckle_rel_depth);
+        hipLaunchKernelGGL(k_cc_border, cc_grid, dim3(256), 0, st, w->items, plane, w->label, p.speckle_rel_depth);
         hipLaunchKernelGGL(k_cc_flatten, grid, dim3(256), 0, st, w->items, w->label, w->size);
         hipLaunchKernelGGL(k_small_keys, grid, dim3(256), 0, st, w->items, w->label, w->size, cost, w->keys, w->counters);
         P_TRY(hipGetLastError());
