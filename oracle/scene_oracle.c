@@ -1,10 +1,12 @@
-/* scene_oracle.c — CPU restatement of the scene preparation (PairStats::new / select /
- * select_diverse, crates/dense/src/neighbors.rs:47-151; depth_ranges / depth_range_of, :162-189)
- * — TEST INFRASTRUCTURE ONLY: the checker of cumulus3d_amd/csrc/neighbors.hip.
+/* scene_oracle.c — CPU restatement of the scene preparation's statistics (PairStats::new,
+ * crates/dense/src/neighbors.rs:47-80; depth_ranges / depth_range_of, :162-189) — TEST
+ * INFRASTRUCTURE ONLY: the checker of cumulus3d_amd/csrc/neighbors.hip. The selections
+ * (select, :83-103) are restated in oracle/__init__.py; select_diverse is checked against a
+ * brute force in tests/test_scene_oracle.py.
  *
  * As the reference: per pair of distinct views a list of f32 angles (appended track by track),
  * sorted in total order when its 75th percentile is asked for; per view a list of positive depths,
- * sorted. libm acos/atan2, double precision.
+ * sorted. libm acos, double precision.
  *
  * Pinning: the reference's tests host_parts.rs:61-89 (selection orders [1,3], [1], [2,1,3], [0,3]
  * and the range 7.5..12.5) and neighbors.rs:194-198 (range 1.5..125 of 1..100), and brute-force
