@@ -84,3 +84,24 @@ def test_graph_replays_equal_eager():
         go, gc, graw, _ = s.densify(sub, SCHED)
         for i in range(len(sub)):
             assert _same(graw[i], sraw[i]) and np.array_equal(gc[i], sc_[i])
+
+
+def test_two_lanes_per_pixel_graph_equals_default():
+    """The mixed batch's schedule captured with two lanes per pixel (tile autotuner's (4, 2)
+    shape) replays to the default eager shape's results bit for bit."""
+    sc, src = _mixed(seed=5)
+    views = list(range(len(sc.views)))
+    be = pm.HipPatchMatch()
+    inp = _input(be, sc, src)
+    with be.begin(inp) as s:
+        eo, ec, eraw, _ = s.densify(views, SCHED)
+    be.close()
+    opts = abi.c3d_pm_options()
+    opts.use_graph, opts.tile_rows, opts.threads_per_pixel = 1, 4, 2
+    be = pm.HipPatchMatch(0, opts)
+    with be.begin(inp) as s:
+        for _ in range(2):
+            go, gc, graw, _ = s.densify(views, SCHED)
+            for v in views:
+                assert _same(graw[v], eraw[v]) and np.array_equal(gc[v], ec[v]) and _same(go[v].normal, eo[v].normal)
+    be.close()
