@@ -105,6 +105,20 @@ def make_batched_workload(nviews, noise, seed):
     return sc, inp, src
 
 
+def make_fast960_workload(nviews, noise, seed, levels_fn):
+    """The reference's published densification setting (fast profile, 3 scales, views of 960 px:
+    the V100 timing of 240 views) on synthetic terrain: a formation flight of `nviews` views at
+    960x540, each with its 10 nearest overlapping views as sources (NeighborParams::num_views),
+    gray pyramids built on the device by `levels_fn`."""
+    cams = scenes.formation_rig(nviews // 3, 3)
+    sc = scenes.make_mixed_scene(cams, [(960, 540)] * len(cams), seed=seed, noise_sigma=noise, noise_seed=seed + 1)
+    src = scenes.nearest_sources(sc, [10] * len(cams))
+    pmp, fl = abi.fast_profile()
+    nl = pm.num_levels(540, 3, 64)
+    inp = scenes.kernel_input(sc, src, [(15.0, 120.0)] * len(cams), pmp, fl, levels=[levels_fn(v, nl) for v in sc.views])
+    return sc, inp, src
+
+
 def cpu_baseline(inp, rows, threads, photometric_iters, view=0):
     """The CPU oracle (C, OpenMP over rows) on a bounded sample: a band of `rows` rows of `view`,
     random-init photometric run (evaluation of the band and its 24-row reach, then the iterations
@@ -279,7 +293,7 @@ def main():
     ap.add_argument("--gpus", type=int, default=1)
     ap.add_argument("--steps", type=int, default=3)
     ap.add_argument("--warmup", type=int, default=1)
-    ap.add_argument("--workload", choices=["zone7", "batched48", "zone108"], default="zone7",
+    ap.add_argument("--workload", choices=["zone7", "batched48", "zone108", "fast960"], default="zone7",
                     help="zone7: 7 views 1920x1080, 6 sources each (default); batched48: 48 views of uneven "
                          "resolution and source count in one batch, schedule captured in a hipGraph")
     ap.add_argument("--width", type=int, default=1920)
@@ -311,22 +325,34 @@ def main():
     if args.workload == "zone108":
         zone_main(args)
         return
-    batched = args.workload == "batched48"
+    batched = args.workload in ("batched48", "fast960")
+    fast960 = args.workload == "fast960"
     use_graph = args.graph if args.graph is not None else int(batched)
     t0 = time.perf_counter()
-    if batched:
+    if fast960:
+        nviews = args.views or 240
+        pbe = pm.HipPatchMatch()
+        sc, inp, src = make_fast960_workload(nviews, 0.01, 6, lambda v, nl: pbe.build_pyramid(v.gray, v.k, nl))
+        pbe.close()
+    elif batched:
         nviews = args.views or 48
         sc, inp, src = make_batched_workload(nviews, 0.01, seed=3)
     else:
         nviews = args.views or 7
         sc, inp, src = make_workload(args.width, args.height, nviews, 0.01, seed=1)
-    sizes = [inp.level_shape(v, 0) for v in range(nviews)]
+    top = inp.num_levels - 1
+    sizes = [inp.level_shape(v, top) for v in range(nviews)]
     npix_v = [h * w for (h, w) in sizes]
     job_mpx = sum(npix_v) / 1e6
     log(f"scene {nviews} views, {job_mpx:.1f} Mpx rendered in {time.perf_counter() - t0:.1f} s")
     views = list(range(nviews))
-    sched = abi.Schedule(photometric_iters=args.photometric_iters, geometric_rounds=args.geometric_rounds,
-                         geometric_iters=args.geometric_iters)
+    if fast960:
+        # LevelSchedule of the fast profile (params.rs:496-503): 6 / 3 photometric iterations,
+        # 2 geometric rounds of 2 per scale
+        sched = abi.Schedule.fast()
+    else:
+        sched = abi.Schedule(photometric_iters=args.photometric_iters, geometric_rounds=args.geometric_rounds,
+                             geometric_iters=args.geometric_iters)
     opts = abi.c3d_pm_options()
     opts.use_graph = use_graph
     be = pm.HipPatchMatch(0, opts)
@@ -335,7 +361,7 @@ def main():
     shape = (4, 1)
     if args.tile_autotune:
         # the half step's workgroup shape (rows, lanes per pixel), outside the timed region
-        shape, shape_ms = sess.autotune(views)
+        shape, shape_ms = sess.autotune(views, level=inp.num_levels - 1)  # the finest scale dominates
         log(f"tile autotuner: (rows, lanes per pixel) {shape}; two iterations (2,1) {shape_ms[0]:.1f} ms, "
             f"(4,1) {shape_ms[1]:.1f} ms, (4,2) {shape_ms[2]:.1f} ms")
     for i in range(args.warmup):
@@ -373,12 +399,15 @@ def main():
     # 58 + 2 (1 + nsrc), plus 8 nsrc for geometric iterations. One launch updates npix/2 pixels
     # of every view.
     wn = (2 * inp.pm.window_radius) // inp.pm.window_step + 1
-    n_photo = 2 * args.photometric_iters
-    n_geo = 2 * args.geometric_iters * args.geometric_rounds
-    bytes_photo = sum(p / 2 * (58 + 2 * (1 + len(s))) for p, s in zip(npix_v, src))
-    bytes_geo = sum(p / 2 * (58 + 2 * (1 + len(s)) + 8 * len(s)) for p, s in zip(npix_v, src))
-    avg_bytes = (n_photo * bytes_photo + n_geo * bytes_geo) / (n_photo + n_geo)
-    flop = sum(FLOP_PER_SAMPLE * 15 * len(s) * wn * wn * p / 2 for p, s in zip(npix_v, src))
+    # every half-step launch of the schedule: (pixels of each view at its scale, geometric?)
+    launches = []
+    for lv in range(inp.num_levels):
+        pix_l = [h * w for (h, w) in (inp.level_shape(v, lv) for v in range(nviews))]
+        n_ph = 2 * (sched.photometric_iters if lv == 0 else sched.photometric_iters_upper)
+        launches += [(pix_l, False)] * n_ph + [(pix_l, True)] * (2 * sched.geometric_rounds * sched.geometric_iters)
+    avg_bytes = sum(sum(p / 2 * (58 + 2 * (1 + len(s)) + (8 * len(s) if geo else 0)) for p, s in zip(pix_l, src))
+                    for pix_l, geo in launches) / len(launches)
+    flop = sum(sum(FLOP_PER_SAMPLE * 15 * len(s) * wn * wn * p / 2 for p, s in zip(pix_l, src)) for pix_l, _ in launches) / len(launches)
     half_ms = sum(t.half_kernel_ms for t in tms) / sum(t.half_launches for t in tms)
     achieved_gbs = avg_bytes / (half_ms / 1e3) / 1e9
     default_zone7 = (not batched and nviews == 7 and (args.width, args.height) == (1920, 1080) and args.photometric_iters == 7
@@ -388,6 +417,12 @@ def main():
         f"{min(len(s) for s in src)}-{max(len(s) for s in src)} src"
     size_desc = f"{sizes[0][1]}x{sizes[0][0]}" if len(set(sizes)) == 1 else \
         "mixed " + "/".join(f"{w}x{h}" for (w, h) in BATCH_SIZES)
+    if fast960:
+        iters_desc = (f"fast profile over {inp.num_levels} scales ({sched.photometric_iters} / {sched.photometric_iters_upper} "
+                      f"photometric, {sched.geometric_rounds}x{sched.geometric_iters} geometric per scale)")
+    else:
+        iters_desc = (f"{args.photometric_iters} photometric + {args.geometric_rounds}x{args.geometric_iters} "
+                      f"geometric iterations")
     line = {
         "metric": "depth-map Mpx/s at <=1% rel. depth error vs plain PatchStereo; HBM GB/s",
         "value": round(value, 3),
@@ -403,14 +438,14 @@ def main():
         "data": "synthetic (seeded fBm terrain, rendered on the host)",
         "config": {
             "workload": f"plain PatchStereo, {nviews} views {size_desc} u8, 1 ref + {nsrc_desc}, "
-                        f"window 5 step 2, {args.photometric_iters} photometric + {args.geometric_rounds}x{args.geometric_iters} "
-                        f"geometric iterations, read-out filter, depth 15-120 m" + (", hipGraph replay" if use_graph else ""),
+                        f"window 5 step 2, {iters_desc}, read-out filter, depth 15-120 m" + (", hipGraph replay" if use_graph else ""),
             "views": nviews,
             "megapixels": round(job_mpx, 3),
-            "sources": [len(s) for s in src] if batched else len(src[0]),
-            "photometric_iters": args.photometric_iters,
-            "geometric_rounds": args.geometric_rounds,
-            "geometric_iters": args.geometric_iters,
+            "sources": [len(s) for s in src] if len({len(s) for s in src}) > 1 else len(src[0]),
+            "photometric_iters": sched.photometric_iters,
+            "geometric_rounds": sched.geometric_rounds,
+            "geometric_iters": sched.geometric_iters,
+            "scales": inp.num_levels,
             "graph": use_graph,
             "tile": {"rows": shape[0], "lanes_per_pixel": shape[1], "autotuned": bool(args.tile_autotune)},
         },
@@ -441,7 +476,7 @@ def main():
         },
     }
     sess.close()
-    if not args.no_cpu:
+    if not args.no_cpu and not fast960:
         threads = min(os.cpu_count() or 1, 16)
         view = max(range(nviews), key=lambda v: npix_v[v])
         rate, dt, sample = cpu_baseline(inp, args.cpu_rows, threads, args.photometric_iters, view)
