@@ -265,3 +265,34 @@ def test_errors_map_to_invalid_argument(small):
         with pytest.raises(pm.PmError) as e:
             s.run([0], _states(inp, [0]), abi.run_params(0, True, False, 1, 1, False), None)
         assert e.value.status == abi.C3D_INVALID_ARGUMENT
+
+
+@pytest.mark.parametrize("sizes", [[(1, 1), (3, 2), (2, 3)], [(13, 7), (130, 5), (7, 40)], [(129, 65), (64, 1), (257, 3)]])
+def test_tiny_and_ragged_views_bit_exact(sizes):
+    """Views smaller than the window, one pixel wide or high, and widths off every tile size, in
+    one batch: a random-init run, a geometric iteration and the read-out equal the oracle's."""
+    import oracle
+
+    cams = scenes.formation_rig(1, 3)
+    sc = scenes.make_mixed_scene(cams, sizes, seed=9)
+    pmp, fl = abi.fast_profile()
+    inp = scenes.kernel_input(sc, SRC3, [(15.0, 120.0)] * 3, pmp, fl)
+    views = [0, 1, 2]
+    be = pm.HipPatchMatch()
+    g = [abi.ViewState(w, h) for (w, h) in sizes]
+    with be.begin(inp) as s:
+        s.run(views, g, abi.run_params(0, False, True, 2, 1, False))
+        snap = abi.DepthSnapshot(3, 0, [x.depth.copy() for x in g])
+        s.run(views, g, abi.run_params(0, True, False, 1, 2, False), snap)
+        gc = s.filter(views, g, snap)
+    be.close()
+    o = oracle.Oracle(inp, threads=4)
+    r = [abi.ViewState(w, h) for (w, h) in sizes]
+    o.run(views, r, abi.run_params(0, False, True, 2, 1, False))
+    rsnap = abi.DepthSnapshot(3, 0, [x.depth.copy() for x in r])
+    o.run(views, r, abi.run_params(0, True, False, 1, 2, False), rsnap)
+    rc = o.filter(views, r, rsnap)
+    for v in views:
+        for a, b in ((g[v].depth, r[v].depth), (g[v].normal, r[v].normal), (g[v].cost, r[v].cost)):
+            assert np.array_equal(np.asarray(a).view(np.uint32), np.asarray(b).view(np.uint32))
+        assert np.array_equal(gc[v], rc[v])
