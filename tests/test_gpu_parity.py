@@ -296,3 +296,41 @@ def test_tiny_and_ragged_views_bit_exact(sizes):
         for a, b in ((g[v].depth, r[v].depth), (g[v].normal, r[v].normal), (g[v].cost, r[v].cost)):
             assert np.array_equal(np.asarray(a).view(np.uint32), np.asarray(b).view(np.uint32))
         assert np.array_equal(gc[v], rc[v])
+
+
+@pytest.mark.parametrize("cams,size", [(12, (96, 64)), (33, (40, 24))])
+def test_many_sources_bit_exact_every_tile_shape(cams, size):
+    """Every view using all others as sources (11, and the maximum of 32): the refinement's
+    source order and early exit, and the LDS tables sized by the source count, under each
+    half-step workgroup shape; states, costs and the read-out equal the oracle's."""
+    import oracle
+
+    rig = scenes.formation_rig(cams // 3, 3)
+    sc = scenes.make_mixed_scene(rig, [size] * cams, seed=11)
+    srcs = [[j for j in range(cams) if j != i] for i in range(cams)]
+    pmp, fl = abi.fast_profile()
+    inp = scenes.kernel_input(sc, srcs, [(15.0, 120.0)] * cams, pmp, fl)
+    assert max(len(s) for s in srcs) <= 32  # C3D_MAX_SOURCES
+    views = list(range(cams))
+
+    def run(engine):
+        st = [abi.ViewState(*size) for _ in views]
+        engine.run(views, st, abi.run_params(0, False, True, 2, 1, False))
+        snap = abi.DepthSnapshot(cams, 0, [x.depth.copy() for x in st])
+        engine.run(views, st, abi.run_params(0, True, False, 1, 2, False), snap)
+        return st, engine.filter(views, st, snap)
+
+    o = oracle.Oracle(inp, threads=16)
+    ref, rc = run(o)
+    for rows, tpp in ((2, 1), (4, 1), (4, 2)):
+        opts = abi.c3d_pm_options()
+        opts.tile_rows, opts.threads_per_pixel = rows, tpp
+        be = pm.HipPatchMatch(0, opts)
+        with be.begin(inp) as s:
+            got, gc = run(s)
+        be.close()
+        for a, b in zip(got, ref):
+            for x, y in ((a.depth, b.depth), (a.normal, b.normal), (a.cost, b.cost)):
+                assert np.array_equal(np.asarray(x).view(np.uint32), np.asarray(y).view(np.uint32)), (rows, tpp)
+        for a, b in zip(gc, rc):
+            assert np.array_equal(a, b), (rows, tpp)
