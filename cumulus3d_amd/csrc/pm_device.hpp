@@ -189,15 +189,14 @@ __device__ __forceinline__ void c3d_sincosf(float x, float& so, float& co) {
     else { so = -c; co = s; }
 }
 
-// Reciprocal of the projective divide of one source sample: bit-pattern seed and three Newton
-// steps (accurate to about 1 ulp for positive normal z, reproducible on any IEEE machine; costs
-// under 2 % of the kernel against v_rcp_f32).
+// Reciprocal of the projective divide of one source sample: bit-pattern seed (relative error
+// below 2^-4.3), one third-order step r(1 + e + e^2) and one Newton step r(1 + e), e = 1 - z r,
+// leaving 2^-25.8 before the last rounding: within 1 ulp of 1/z for positive normal z (0.78 ulp
+// at most over a binade), reproducible on any IEEE machine, 6 VALU operations.
 __device__ __forceinline__ float rcp_sample(float z) {
     float r = __uint_as_float(0x7EF311C3u - __float_as_uint(z));
     float e = fmaf(-z, r, 1.0f);
-    r = fmaf(r, e, r);
-    e = fmaf(-z, r, 1.0f);
-    r = fmaf(r, e, r);
+    r = fmaf(r, fmaf(e, e, e), r);
     e = fmaf(-z, r, 1.0f);
     r = fmaf(r, e, r);
     return r;

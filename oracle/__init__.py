@@ -71,6 +71,7 @@ def load_library(path: str):
         L.pmo_bilinear_clamped.argtypes = [C.POINTER(C.c_float), C.c_size_t, C.c_size_t, C.c_float, C.c_float]
         L.pmo_bilinear_clamped.restype = C.c_float
         L.pmo_set_rcp_mode.argtypes = [C.c_int]
+        L.pmo_rcp_sample.argtypes = [C.POINTER(C.c_float), C.POINTER(C.c_float), C.c_size_t]
         L.pmo_expf.argtypes = [C.c_float]
         L.pmo_expf.restype = C.c_float
         L.pmo_sincosf.argtypes = [C.c_float, C.POINTER(C.c_float), C.POINTER(C.c_float)]
@@ -115,6 +116,15 @@ def load_library(path: str):
                   "pmo_remove_speckles", "pmo_fill_holes", "pmo_pair_stats", "pmo_depth_ranges"):
             getattr(L, n).restype = C.c_int
         return L
+
+
+def rcp_sample(z: np.ndarray) -> np.ndarray:
+    """The device's per-sample reciprocal of float32 values."""
+    z = np.ascontiguousarray(z, np.float32)
+    r = np.empty_like(z)
+    fp = C.POINTER(C.c_float)
+    lib().pmo_rcp_sample(z.ctypes.data_as(fp), r.ctypes.data_as(fp), z.size)
+    return r
 
 
 def build_pyramid(gray: np.ndarray, k, levels: int):

@@ -251,18 +251,24 @@ void pmo_sincosf(float x, float* so, float* co) {
     else { *so = -c; *co = s; }
 }
 
-/* reciprocal of a positive normal float: bit-pattern seed and three Newton steps (mode 1), or
- * IEEE division (mode 0) */
-static inline float rcp_sample(float z) {
-    if (g_rcp_mode == 0) return 1.0f / z;
+/* reciprocal of a positive normal float: bit-pattern seed, one third-order step r(1 + e + e^2)
+ * and one Newton step r(1 + e) with e = 1 - z r (mode 1), or IEEE division (mode 0) */
+static inline float rcp_seeded(float z) {
     float r = bits_f(0x7EF311C3u - f_bits(z));
     float e = fmaf(-z, r, 1.0f);
-    r = fmaf(r, e, r);
-    e = fmaf(-z, r, 1.0f);
-    r = fmaf(r, e, r);
+    r = fmaf(r, fmaf(e, e, e), r);
     e = fmaf(-z, r, 1.0f);
     r = fmaf(r, e, r);
     return r;
+}
+
+void pmo_rcp_sample(const float* z, float* r, size_t n) {
+    for (size_t i = 0; i < n; ++i) r[i] = rcp_seeded(z[i]);
+}
+
+static inline float rcp_sample(float z) {
+    if (g_rcp_mode == 0) return 1.0f / z;
+    return rcp_seeded(z);
 }
 
 /* ================================================================ geometry */

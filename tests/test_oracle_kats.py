@@ -158,6 +158,20 @@ def test_sincos_matches_libm(oracle_lib):
     assert worst < 2e-7
 
 
+def test_rcp_sample_within_one_ulp_of_division(oracle_lib):
+    """The device's per-sample reciprocal against IEEE float32 division: every float of [1, 2)
+    (the seed and both steps act on the significand alone) and 10^6 values over [2^-40, 2^40]
+    within 1 ulp; at least 90 % correctly rounded."""
+    z = np.arange(0x3F800000, 0x40000000, dtype=np.uint32).view(np.float32)
+    rng = np.random.default_rng(5)
+    z = np.concatenate([z, np.exp2(rng.uniform(-40, 40, 10**6)).astype(np.float32)])
+    got = oracle_lib.rcp_sample(z)
+    want = np.float32(1.0) / z
+    d = np.abs(got.view(np.int32).astype(np.int64) - want.view(np.int32).astype(np.int64))
+    assert d.max() <= 1
+    assert (d == 0).mean() > 0.9
+
+
 def _scene_and_input(w=160, h=96, seed=3):
     rig = scenes.oblique_rig(3, baseline=6.0)
     sc = scenes.make_terrain_scene(rig, w, h, focal=500.0 * w / 640.0, seed=seed)
