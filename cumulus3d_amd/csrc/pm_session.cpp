@@ -1073,7 +1073,11 @@ extern "C" {
 
 int c3d_pm_densify(c3d_pm_session* s, const uint32_t* views, uint32_t n, const c3d_schedule* sch, c3d_view_state* out,
                    uint8_t* const* counts, float* const* raw_depth, c3d_timing* timing) {
-    if (!s || !sch || !views || n == 0) return fail(C3D_INVALID_ARGUMENT, "null argument");
+    if (!s || !sch || (n && !views)) return fail(C3D_INVALID_ARGUMENT, "null argument");
+    if (n == 0) {  // nothing to estimate (densify.rs:290-425 loops over no views)
+        if (timing) *timing = c3d_timing{};
+        return C3D_OK;
+    }
     int rc = set_device(s->be);
     if (rc) return rc;
     const uint32_t top = (uint32_t)s->nl - 1;
@@ -1183,7 +1187,11 @@ int c3d_pm_densify(c3d_pm_session* s, const uint32_t* views, uint32_t n, const c
 int c3d_pm_densify_learned(c3d_pm_session* s, c3d_init_net* net, const uint32_t* views, uint32_t n, const c3d_learned_schedule* ls,
                            c3d_view_state* out, uint8_t* const* counts, float* const* raw_depth, float* const* init_depth,
                            float* const* init_normal, c3d_timing* timing) {
-    if (!s || !net || !ls || !views || n == 0) return fail(C3D_INVALID_ARGUMENT, "null argument");
+    if (!s || !net || !ls || (n && !views)) return fail(C3D_INVALID_ARGUMENT, "null argument");
+    if (n == 0) {
+        if (timing) *timing = c3d_timing{};
+        return C3D_OK;
+    }
     const nn::NetView nv = nn::net_view(net);
     if (nv.be != s->be) return fail(C3D_INVALID_ARGUMENT, "the network belongs to another backend");
     int rc = check_post(ls->post, ls->post_k);
@@ -1411,7 +1419,8 @@ int c3d_pm_densify_learned(c3d_pm_session* s, c3d_init_net* net, const uint32_t*
 
 int c3d_pm_autotune(c3d_pm_session* s, const uint32_t* views, uint32_t n, uint32_t level, uint32_t* rows_out, uint32_t* tpp_out,
                     double* ms_out) {
-    if (!s || !views || n == 0) return fail(C3D_INVALID_ARGUMENT, "null argument");
+    if (!s || !views) return fail(C3D_INVALID_ARGUMENT, "null argument");
+    if (n == 0) return fail(C3D_INVALID_ARGUMENT, "no views to time");
     int rc = set_device(s->be);
     if (rc) return rc;
     rc = check_views(s, level, views, n, nullptr);

@@ -122,7 +122,7 @@ typedef struct c3d_learned_schedule {
  * sources, plane-sweep features, the network over batches of `net`'s size, planes at full
  * resolution, then the schedule above. The network's input size must hold the quarter-resolution
  * images. Outputs as c3d_pm_densify; init_depth / init_normal (optional, per view) receive the
- * initial planes (depth 0 = random initialisation). */
+ * initial planes (depth 0 = random initialisation). No views is a no-op. */
 int c3d_pm_densify_learned(c3d_pm_session* session, c3d_init_net* net, const uint32_t* views, uint32_t num_views,
                            const c3d_learned_schedule* schedule, c3d_view_state* out, uint8_t* const* counts,
                            float* const* raw_depth, float* const* init_depth, float* const* init_normal,

@@ -310,7 +310,8 @@ typedef struct c3d_timing {
 /* Runs the schedule for `views`; `out`, `counts` and `raw_depth` are at the top scale
  * (num_levels - 1). When `out` is not NULL the final states are written: depth/normal zeroed where the threshold fails, raw cost kept; when
  * `counts` is not NULL the filter counts are written. With both NULL the results stay on the
- * device (for throughput measurement). `timing` may be NULL. */
+ * device (for throughput measurement). `timing` may be NULL. No views (num_views 0, `views` may
+ * then be NULL) is a no-op. */
 int c3d_pm_densify(c3d_pm_session* session, const uint32_t* views, uint32_t num_views,
                    const c3d_schedule* schedule, c3d_view_state* out, uint8_t* const* counts,
                    float* const* raw_depth, c3d_timing* timing);

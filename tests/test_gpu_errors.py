@@ -77,3 +77,28 @@ def test_densify_post_needs_intrinsics(be):
         # the session stays usable
         outs, _, _, tm = s.densify([0, 1, 2], abi.Schedule(photometric_iters=1, geometric_rounds=0, post=abi.PostParams.fast()))
         assert all(o.depth.shape == (48, 64) for o in outs)
+
+
+def test_empty_view_lists_are_noops(be):
+    """No views: run, evaluate, filter, upsample, median and densify do nothing and succeed (the
+    reference's batches and schedule loop over no views, patchmatch.rs:606-625, densify.rs:290-425);
+    the autotuner, which has nothing to time, refuses; the session stays usable."""
+    rig = scenes.oblique_rig(3, baseline=6.0)
+    sc = scenes.make_terrain_scene(rig, 128, 96, focal=100.0, seed=3)
+    pmp, fl = abi.fast_profile()
+    levels = [be.build_pyramid(v.gray, v.k, 2) for v in sc.views]
+    inp = scenes.kernel_input(sc, [[1, 2], [0, 2], [0, 1]], [(20.0, 80.0)] * 3, pmp, fl, levels=levels)
+    assert inp.num_levels == 2
+    snap = abi.DepthSnapshot(1, 1, [None, None, None])
+    with be.begin(inp) as s:
+        s.run([], [], abi.run_params(0, False, True, 1, 1, False))
+        assert s.evaluate(1, [], []) == []
+        assert s.filter([], [], snap) == []
+        assert s.upsample(1, [], [], 1.0, 10.0) == []
+        assert s.median_filter(1, [], []) == []
+        outs, counts, raws, tm = s.densify([], abi.Schedule(photometric_iters=1, geometric_rounds=0))
+        assert outs == [] and counts == [] and raws == [] and tm.total_ms == 0.0
+        st, msg = _status(lambda: s.autotune([]))
+        assert st == abi.C3D_INVALID_ARGUMENT and "no views" in msg
+        outs, _, _, _ = s.densify([0, 1, 2], abi.Schedule(photometric_iters=1, geometric_rounds=0))
+        assert all(o.depth.shape == (96, 128) for o in outs)
