@@ -12,7 +12,7 @@
 //
 // Arithmetic: every operation is written out (explicit fmaf where an FMA is intended; the library
 // is compiled with -ffp-contract=off) in the order the CPU oracle uses, with IEEE division and
-// square root, the polynomial exp / sin-cos below and a Newton reciprocal for the per-sample
+// square root, the polynomial exp / sin-cos below and a seeded reciprocal for the per-sample
 // projective divide, so that the device and the CPU oracle agree bit for bit.
 #pragma once
 

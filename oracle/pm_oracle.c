@@ -20,8 +20,8 @@
  * in gray levels (0..255) and shifted by the reference centre value before the NCC sums, a
  * two-pass weighted reference variance, and the covariance as sum(w s' I)/W - mu mean(s'). These are exact algebraic restatements of the reference's
  * formulas (NCC is invariant to the 1/255 scale and to the shift); they change only rounding.
- * The per-sample reciprocal of the projective divide is the device's three-step Newton reciprocal
- * (mode 1, the default); mode 0 (IEEE division) is kept to measure what that choice changes.
+ * The per-sample reciprocal of the projective divide is the device's seeded reciprocal (one
+ * third-order and one Newton step; mode 1, the default); mode 0 (IEEE division) is kept to measure what that choice changes.
  *
  * Pinning (what fixes each part independently of this file):
  *   hash_bytes, mix64, rng_*     published FNV-1a-64 / splitmix64 vectors (tests/test_oracle_kats.py)
@@ -30,6 +30,8 @@
  *   bilinear_clamped             image.rs:316-326 (ramp value 26.25)
  *   num_levels, build_pyramid    upsample.rs:266-276 (levels 3/2/1, cx = 1.75, constant image)
  *   pmo_expf / pmo_sincosf       libm to 2 ulp over their domains
+ *   rcp_seeded                   IEEE division to 1 ulp over every float of [1, 2) and 10^6
+ *                                values over [2^-40, 2^40]
  *   level_pair + homography      brute-force projection of the plane's 3-D point (1e-4 px)
  *   ncc_cost (FP32)              FP64 cost oracle below to 1e-4; closed forms: identical
  *                                textures under the true plane give cost 0, constant patch 2,
@@ -69,7 +71,7 @@ static __thread char g_err[512];
 static void set_err(const char* m) { snprintf(g_err, sizeof g_err, "%s", m); }
 const char* pmo_last_error(void) { return g_err; }
 
-static int g_rcp_mode = 1; /* 0: IEEE 1/z, 1: Newton reciprocal (the device's) */
+static int g_rcp_mode = 1; /* 0: IEEE 1/z, 1: seeded reciprocal (the device's) */
 void pmo_set_rcp_mode(int mode) { g_rcp_mode = mode; }
 
 /* ================================================================ integer primitives */

@@ -1,17 +1,17 @@
 """GPU parity of the gfx950 engine against the CPU oracle, through the C ABI.
 
-Workload: config 1 of the reference's hot path — seeded fBm terrain (200 x 200 m, +-15 m relief,
+Workload: 3 oblique views of 640 x 360 — seeded fBm terrain (200 x 200 m, +-15 m relief,
 random albedo) imaged by 3 pinhole cameras 30 m up at 45 deg oblique, f = 500 px, 640 x 360 u8,
 each view using the other two as sources, window radius 5 step 2, 5 iterations, depth 20-80 m.
 
 Bars:
-  * per-hypothesis NCC cost within 1e-4 absolute of the FP64 evaluation (north_star tolerance);
+  * per-hypothesis NCC cost within 1e-4 absolute of the FP64 evaluation;
   * the engine equals the FP32 oracle bit for bit for costs, runs, evaluation, read-out,
     upsampling, the median filter and the whole device-resident schedule;
   * against the oracle computing the per-sample projective divide with IEEE division (the
-    reference's literal formula; the engine uses a reproducible Newton reciprocal) depth agrees
+    reference's literal formula; the engine uses a reproducible seeded reciprocal) depth agrees
     within 1 % relative on >= 99 % of valid pixels and normals within 2 deg on >= 99 %
-    (north_star tolerance).
+    (the accuracy gate of the depth-map metric).
 """
 import numpy as np
 import pytest
@@ -119,7 +119,7 @@ def _agreement(d_gpu, d_ref, n_gpu, n_ref):
 
 
 def test_run_bit_exact(c1):
-    """Random-init photometric run, 5 iterations (config 1): identical to the oracle."""
+    """Random-init photometric run, 5 iterations (3 views of 640 x 360): identical to the oracle."""
     sc, inp = c1
     be = pm.HipPatchMatch()
     o = _oracle(inp, 1)
