@@ -342,9 +342,10 @@ __device__ __forceinline__ float quad_sample(const uint32_t* qbase, float qp4, f
     return fmaf(ay, bot - top, top);
 }
 
-// Gathers kept in flight by the window loop. Measured on 7 x 1920x1080 with 6 sources (job time):
-// 4 -> 502.8 ms, 6 -> 500.8 ms, 8 -> 500.8 ms (255 VGPRs), 12 spills; the compiler's own batches of
-// eight gathers drained to zero before the next batch gave 512.9 ms.
+// Gathers kept in flight by the window loop (the default; the half step picks its own). Measured
+// on 7 x 1920x1080 with 6 sources (job time): 4 -> 502.8 ms, 6 -> 500.8 ms, 8 -> 500.8 ms
+// (255 VGPRs: photometric half steps 2.9 ms faster, geometric ones 2.8 ms slower), 12 spills; the
+// compiler's own batches of eight gathers drained to zero before the next batch gave 512.9 ms.
 constexpr int WINDOW_PIPE = 6;
 
 // Window sums of one hypothesis x source (H translated to the pixel). A sample whose projective
@@ -352,7 +353,7 @@ constexpr int WINDOW_PIPE = 6;
 // four window corners have z > 1e-8: z is monotone in each offset (every step is a correctly
 // rounded affine operation), so then every sample does. With CHECK the sample is computed anyway
 // (any position is clamped into the quad table) and replaced by a select.
-template <int WR, int WSTEP, bool CHECK, class REF>
+template <int WR, int WSTEP, bool CHECK, int PIPE, class REF>
 __device__ __forceinline__ void window_sums(const RefWin<WR, WSTEP>& rw, const REF& ref, int x, int y, const uint32_t* qbase, float qp4,
                                             float koff4, float w1, float h1, const float* H, float& Ss, float& SSs, float& IS) {
     constexpr int WN = Win<WR, WSTEP>::WN;
@@ -363,7 +364,6 @@ __device__ __forceinline__ void window_sums(const RefWin<WR, WSTEP>& rw, const R
     // before sample k is consumed, so PIPE gathers stay in flight. The sums take the samples in
     // row-major order as before.
     constexpr int N = WN * WN;
-    constexpr int PIPE = WINDOW_PIPE;
     float fx[N], fy[N];
     uint32_t qv[N];
     auto issue = [&](int k) {
@@ -405,7 +405,7 @@ __device__ __forceinline__ void window_sums(const RefWin<WR, WSTEP>& rw, const R
 
 // Bilateral-weighted NCC cost 1 - rho, 2 on failure (patchmatch.cu:244-290); the covariance is
 // sum(w s' I)/W - mu mean(s') with s' the shifted source samples.
-template <int WR, int WSTEP, class REF>
+template <int WR, int WSTEP, int PIPE = WINDOW_PIPE, class REF>
 __device__ __forceinline__ float ncc_cost(const KParams& P, const RefWin<WR, WSTEP>& rw, const REF& ref, const LevelDesc& S,
                                           const float* H, int x, int y) {
     float z = H[8];
@@ -423,8 +423,8 @@ __device__ __forceinline__ float ncc_cost(const KParams& P, const RefWin<WR, WST
     const bool all_front = fmaf(H[6], lo, za) > 1e-8f && fmaf(H[6], hi, za) > 1e-8f && fmaf(H[6], lo, zb) > 1e-8f &&
                            fmaf(H[6], hi, zb) > 1e-8f;
     float Ss, SSs, IS;
-    if (all_front) window_sums<WR, WSTEP, false>(rw, ref, x, y, qbase, qp4, koff4, w1, h1, H, Ss, SSs, IS);
-    else window_sums<WR, WSTEP, true>(rw, ref, x, y, qbase, qp4, koff4, w1, h1, H, Ss, SSs, IS);
+    if (all_front) window_sums<WR, WSTEP, false, PIPE>(rw, ref, x, y, qbase, qp4, koff4, w1, h1, H, Ss, SSs, IS);
+    else window_sums<WR, WSTEP, true, PIPE>(rw, ref, x, y, qbase, qp4, koff4, w1, h1, H, Ss, SSs, IS);
     const float iW = rw.iW;
     float mus = Ss * iW;
     float vars = fmaf(-mus, mus, SSs * iW);
@@ -548,10 +548,10 @@ __device__ __forceinline__ void ref_stats_any(const KParams& P, const REF& ref, 
     else ref_stats_dyn<WSTEP>(P, ref, x, y, wr, rw);
 }
 
-template <int WR, int WSTEP, class REF>
+template <int WR, int WSTEP, int PIPE = WINDOW_PIPE, class REF>
 __device__ __forceinline__ float ncc_cost_any(const KParams& P, const RefWinT<WR, WSTEP>& rw, const REF& ref, const LevelDesc& S,
                                               const float* H, int x, int y) {
-    if constexpr (WR > 0) return ncc_cost<WR, WSTEP>(P, rw, ref, S, H, x, y);
+    if constexpr (WR > 0) return ncc_cost<WR, WSTEP, PIPE>(P, rw, ref, S, H, x, y);
     else return ncc_cost_dyn<WSTEP>(P, rw, ref, S, H, x, y);
 }
 

@@ -54,13 +54,13 @@ __device__ __forceinline__ TileRefT<WR, TW> tile_ref(const float* tile, int x0, 
 
 // (photometric cost, truncated geometric error) of one hypothesis x one source with the pixel's
 // reciprocal focal lengths hoisted (patchmatch.cu:316-324)
-template <int WR, int WSTEP, bool GEO, class REF>
+template <int WR, int WSTEP, bool GEO, int PIPE, class REF>
 __device__ __forceinline__ float pair_cost_h(const KParams& P, const RefWinT<WR, WSTEP>& rw, const REF& ref, const LevelDesc& R,
                                              const LevelDesc& S, const PairRec& pr, const LevelPair& lp, float ifx, float ify, int x,
                                              int y, float4 h, float& geo) {
     float H[9];
     float c = 2.0f;
-    if (homography(R, lp, ifx, ify, (float)x, (float)y, h.w, xyz(h), H)) c = ncc_cost_any<WR, WSTEP>(P, rw, ref, S, H, x, y);
+    if (homography(R, lp, ifx, ify, (float)x, (float)y, h.w, xyz(h), H)) c = ncc_cost_any<WR, WSTEP, PIPE>(P, rw, ref, S, H, x, y);
     geo = GEO ? fminf(geo_error(P, R, S, pr, x, y, h.w), P.gmax) : 0.f;
     return c;
 }
@@ -75,6 +75,10 @@ template <int WR, int WSTEP, bool GEO, int ROWS, int TPP>
 __global__ __launch_bounds__(ROWS * 64) __attribute__((amdgpu_waves_per_eu(2, 2))) void k_half(KParams P) {
     constexpr int NT = ROWS * 64;
     constexpr int SPAN = HALF_SPAN / TPP;  // pixel span of one wave row (64 / TPP active pixels)
+    // source gathers in flight in the window loop: eight where the registers allow (photometric,
+    // one lane per pixel: 255 VGPRs), six otherwise, fewer for the 49 register-held weights of a
+    // 7 x 7 window (deeper pipelines spill there)
+    constexpr int PIPE = Win<(WR > 0 ? WR : 1), WSTEP>::WN2 > 36 ? 3 : ((GEO || TPP > 1) ? 6 : 8);
     constexpr int TW = SPAN + 2 * WR;
     constexpr int TH = ROWS + 2 * WR;
     __shared__ float tile_s[WR > 0 ? TW * TH : 1];
@@ -192,7 +196,7 @@ __global__ __launch_bounds__(ROWS * 64) __attribute__((amdgpu_waves_per_eu(2, 2)
 #pragma nounroll
         for (int i = sub; i < 9; i += TPP) {
             float gi;
-            const float mi = pair_cost_h<WR, WSTEP, GEO>(P, rw, ref, R, S, pr, lp, ifx, ify, x, y, hyp[i * NT], gi);
+            const float mi = pair_cost_h<WR, WSTEP, GEO, PIPE>(P, rw, ref, R, S, pr, lp, ifx, ify, x, y, hyp[i * NT], gi);
 #pragma unroll
             for (int q = 0; q < 9; ++q)
                 if (q == i) {
@@ -314,7 +318,7 @@ __global__ __launch_bounds__(ROWS * 64) __attribute__((amdgpu_waves_per_eu(2, 2)
                 const PairRec& pr = prs[j];
                 const LevelDesc S = P.lv[pr.src * P.nlevels + P.level];
                 float g;
-                float m = pair_cost_h<WR, WSTEP, GEO>(P, rw, ref, R, S, pr, lps[j], ifx, ify, x, y, hc, g);
+                float m = pair_cost_h<WR, WSTEP, GEO, PIPE>(P, rw, ref, R, S, pr, lps[j], ifx, ify, x, y, hc, g);
                 nm += wj * (m + P.lambda * g);
                 if (early && nm / den >= bcost) {
                     out = true;
