@@ -80,11 +80,12 @@ def copy_calibration(mib):
     log(f"copy calibration: {mib} MiB device-to-device")
 
 
-def make_workload(width, height, nviews, noise, seed):
+def make_workload(width, height, nviews, noise, seed, window=(5, 2)):
     rig = scenes.oblique_rig(nviews, baseline=6.0)
     sc = scenes.make_terrain_scene(rig, width, height, focal=500.0, seed=seed, noise_sigma=noise, noise_seed=seed + 1)
     src = [[j for j in range(nviews) if j != v] for v in range(nviews)]
     pmp, fl = abi.fast_profile()
+    pmp = abi.PmParams(**{**pmp.__dict__, "window_radius": window[0], "window_step": window[1]})
     inp = scenes.kernel_input(sc, src, [(15.0, 120.0)] * nviews, pmp, fl)
     return sc, inp, src
 
@@ -296,6 +297,8 @@ def main():
     ap.add_argument("--workload", choices=["zone7", "batched48", "zone108", "fast960"], default="zone7",
                     help="zone7: 7 views 1920x1080, 6 sources each (default); batched48: 48 views of uneven "
                          "resolution and source count in one batch, schedule captured in a hipGraph")
+    ap.add_argument("--window", type=int, nargs=2, default=[5, 2], metavar=("RADIUS", "STEP"),
+                    help="default workload: NCC window radius and step (5 2 = fast profile, 5 1 = quality profile's window)")
     ap.add_argument("--width", type=int, default=1920)
     ap.add_argument("--height", type=int, default=1080)
     ap.add_argument("--views", type=int, default=None)
@@ -339,7 +342,7 @@ def main():
         sc, inp, src = make_batched_workload(nviews, 0.01, seed=3)
     else:
         nviews = args.views or 7
-        sc, inp, src = make_workload(args.width, args.height, nviews, 0.01, seed=1)
+        sc, inp, src = make_workload(args.width, args.height, nviews, 0.01, seed=1, window=tuple(args.window))
     top = inp.num_levels - 1
     sizes = [inp.level_shape(v, top) for v in range(nviews)]
     npix_v = [h * w for (h, w) in sizes]
@@ -411,8 +414,12 @@ def main():
     half_ms = sum(t.half_kernel_ms for t in tms) / sum(t.half_launches for t in tms)
     achieved_gbs = avg_bytes / (half_ms / 1e3) / 1e9
     default_zone7 = (not batched and nviews == 7 and (args.width, args.height) == (1920, 1080) and args.photometric_iters == 7
-                     and args.geometric_rounds == 1 and args.geometric_iters == 2)
+                     and args.geometric_rounds == 1 and args.geometric_iters == 2 and tuple(args.window) == (5, 2))
     achieved_tflops = flop / (half_ms / 1e3) / 1e12
+    # the windows built with register-held weights; every other one runs the any-radius build
+    reg_windows = {(5, 2), (4, 2), (3, 2), (2, 2), (3, 1), (2, 1), (1, 1)}
+    kernel_name = (f"k_half<{inp.pm.window_radius},{inp.pm.window_step}>"
+                   if (inp.pm.window_radius, inp.pm.window_step) in reg_windows else f"k_half<any radius,{inp.pm.window_step}>")
     nsrc_desc = f"{len(src[0])} src each" if len({len(s) for s in src}) == 1 else \
         f"{min(len(s) for s in src)}-{max(len(s) for s in src)} src"
     size_desc = f"{sizes[0][1]}x{sizes[0][0]}" if len(set(sizes)) == 1 else \
@@ -438,7 +445,8 @@ def main():
         "data": "synthetic (seeded fBm terrain, rendered on the host)",
         "config": {
             "workload": f"plain PatchStereo, {nviews} views {size_desc} u8, 1 ref + {nsrc_desc}, "
-                        f"window 5 step 2, {iters_desc}, read-out filter, depth 15-120 m" + (", hipGraph replay" if use_graph else ""),
+                        f"window {inp.pm.window_radius} step {inp.pm.window_step}, {iters_desc}, read-out filter, depth 15-120 m"
+                        + (", hipGraph replay" if use_graph else ""),
             "views": nviews,
             "megapixels": round(job_mpx, 3),
             "sources": [len(s) for s in src] if len({len(s) for s in src}) > 1 else len(src[0]),
@@ -451,7 +459,7 @@ def main():
         },
         "roofline": {
             "bound": "hbm",
-            "kernel": "k_half<5,2>",
+            "kernel": kernel_name,
             "achieved": round(achieved_gbs, 2),
             "peak": HBM_PEAK_GBS,
             "unit": "GB/s",
@@ -462,7 +470,7 @@ def main():
         },
         "roofline_valu": {
             "bound": "valu-fp32",
-            "kernel": "k_half<5,2>",
+            "kernel": kernel_name,
             "achieved": round(achieved_tflops, 2),
             "peak": FP32_PEAK_TFLOPS,
             "unit": "TFLOP/s",
