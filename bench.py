@@ -46,7 +46,7 @@ HBM_PEAK_GBS = 8000.0
 # iterations), averaged over its 18 launches: rocprofv3 FETCH_SIZE x 2.0 + WRITE_SIZE x 1.0 (factors
 # from a 512 MiB copy in the same run), profiles/k_half-zone7-7x1080p-6src.md. FETCH_SIZE counts L2
 # misses, which the Infinity Cache mostly serves: an upper bound on HBM reads.
-K_HALF_TRAFFIC_ZONE7 = 26.084e9
+K_HALF_TRAFFIC_ZONE7 = 25.995e9
 FP32_PEAK_TFLOPS = 157.3
 FLOP_PER_SAMPLE = 28  # projective transform 9, reciprocal 1, bilinear 7, scale 1, accumulations 6 (rounded up)
 
