@@ -102,3 +102,20 @@ def test_empty_view_lists_are_noops(be):
         assert st == abi.C3D_INVALID_ARGUMENT and "no views" in msg
         outs, _, _, _ = s.densify([0, 1, 2], abi.Schedule(photometric_iters=1, geometric_rounds=0))
         assert all(o.depth.shape == (96, 128) for o in outs)
+
+
+def test_image_size_limit(be):
+    """Source-table byte offsets are formed exactly in FP32, so views are limited to
+    (w + 3)(h + 3) <= 2^24 pixels: 4093 x 4093 opens, 4096 x 4096 is refused at begin."""
+    rig = scenes.oblique_rig(3, baseline=6.0)
+    sc = scenes.make_terrain_scene(rig, 64, 48, focal=50.0, seed=3)
+    pmp, fl = abi.fast_profile()
+    for side, ok in ((4093, True), (4096, False)):
+        big = np.zeros((side, side), np.uint8)
+        levels = [[abi.LevelData(big, [50.0, 50.0, side / 2.0, side / 2.0])] for _ in sc.views]
+        inp = scenes.kernel_input(sc, [[1, 2], [0, 2], [0, 1]], [(20.0, 80.0)] * 3, pmp, fl, levels=levels)
+        if ok:
+            be.begin(inp).close()
+        else:
+            st, msg = _status(lambda: be.begin(inp))
+            assert st == abi.C3D_INVALID_ARGUMENT and "2^24" in msg
