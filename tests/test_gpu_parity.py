@@ -334,3 +334,36 @@ def test_many_sources_bit_exact_every_tile_shape(cams, size):
                 assert np.array_equal(np.asarray(x).view(np.uint32), np.asarray(y).view(np.uint32)), (rows, tpp)
         for a, b in zip(gc, rc):
             assert np.array_equal(a, b), (rows, tpp)
+
+
+def test_pair_costs_bit_exact_at_the_maximum_image_size():
+    """Views of 4093 x 4093, the largest the source tables address exactly ((w + 3)(h + 3) = 2^24):
+    per-hypothesis costs at pixels near every corner and in the middle, random planes, equal the
+    oracle's bit for bit (uniform-noise images; the geometry of the 3-view oblique rig)."""
+    import oracle
+
+    side = 4093
+    rig = scenes.oblique_rig(3, baseline=6.0)
+    sc = scenes.make_terrain_scene(rig, 64, 48, focal=50.0, seed=3)
+    rng = np.random.default_rng(17)
+    f = 50.0 * side / 64.0
+    levels = [[abi.LevelData(rng.integers(0, 256, (side, side), dtype=np.uint8), [f, f, (side - 1) / 2.0, (side - 1) / 2.0])]
+              for _ in sc.views]
+    pmp, fl = abi.fast_profile()
+    inp = scenes.kernel_input(sc, SRC3, [(20.0, 80.0)] * 3, pmp, fl, levels=levels)
+    n = 1500
+    edge = np.concatenate([rng.integers(0, 64, n // 3), rng.integers(side - 64, side, n // 3),
+                           rng.integers(0, side, n - 2 * (n // 3))])
+    xy = np.stack([edge, rng.permutation(edge)], 1).astype(np.int32)
+    nrm = rng.normal(size=(n, 3))
+    nrm[:, 2] = -np.abs(nrm[:, 2]) - 0.5
+    nrm /= np.linalg.norm(nrm, axis=1, keepdims=True)
+    planes = np.concatenate([nrm, rng.uniform(20.0, 80.0, (n, 1))], 1).astype(np.float32)
+    be = pm.HipPatchMatch()
+    with be.begin(inp) as s:
+        got = s.pair_costs(0, 0, 1, xy, planes)
+    be.close()
+    o = oracle.Oracle(inp, threads=16)
+    ref = np.array([o.ncc_cost_f32(0, 0, 1, int(x), int(y), p) for (x, y), p in zip(xy, planes)], np.float32)
+    assert (ref < 2.0).sum() > n // 10  # enough windows inside the source to test its far addresses
+    assert np.array_equal(got.view(np.uint32), ref.view(np.uint32))
