@@ -367,3 +367,47 @@ def test_pair_costs_bit_exact_at_the_maximum_image_size():
     ref = np.array([o.ncc_cost_f32(0, 0, 1, int(x), int(y), p) for (x, y), p in zip(xy, planes)], np.float32)
     assert (ref < 2.0).sum() > n // 10  # enough windows inside the source to test its far addresses
     assert np.array_equal(got.view(np.uint32), ref.view(np.uint32))
+
+
+@pytest.mark.parametrize("random_init", [0, 2])
+def test_caller_states_with_nan_zero_and_negative_depths(small, random_init):
+    """Caller-provided states holding NaN, zero and negative depths and NaN normals, run without
+    random initialisation (such planes cost 2 and are replaced by propagation) and with
+    random_init = 2 (only pixels without a positive depth are re-initialised): 2 iterations and
+    the read-out equal the oracle's bit for bit."""
+    import oracle
+
+    sc, inp = small
+    views = [0, 1, 2]
+
+    def states():
+        out = []
+        for v in views:
+            h, w = inp.level_shape(v, 0)
+            st = abi.ViewState(w, h)
+            st.depth[...] = np.asarray(sc.gt_depth[v], np.float32) * np.float32(1.03)
+            st.normal[...] = np.asarray(sc.gt_normal[v], np.float32)
+            r = np.random.default_rng(100 + v).random((h, w))
+            st.depth[r < 0.05] = np.nan
+            st.depth[(r >= 0.05) & (r < 0.10)] = 0.0
+            st.depth[(r >= 0.10) & (r < 0.15)] = -5.0
+            st.normal[(r >= 0.15) & (r < 0.18)] = np.nan
+            out.append(st)
+        return out
+
+    p = abi.run_params(0, False, random_init, 2, 3, False)
+    be = pm.HipPatchMatch()
+    g = states()
+    with be.begin(inp) as s:
+        s.run(views, g, p)
+        snap = abi.DepthSnapshot(7, 0, [x.depth.copy() for x in g])
+        gc = s.filter(views, g, snap)
+    be.close()
+    o = oracle.Oracle(inp, threads=16)
+    r = states()
+    o.run(views, r, p)
+    rc = o.filter(views, r, abi.DepthSnapshot(7, 0, [x.depth.copy() for x in r]))
+    for v in views:
+        for a, b in ((g[v].depth, r[v].depth), (g[v].normal, r[v].normal), (g[v].cost, r[v].cost)):
+            assert np.array_equal(np.asarray(a).view(np.uint32), np.asarray(b).view(np.uint32))
+        assert np.array_equal(gc[v], rc[v])
