@@ -411,3 +411,35 @@ def test_caller_states_with_nan_zero_and_negative_depths(small, random_init):
         for a, b in ((g[v].depth, r[v].depth), (g[v].normal, r[v].normal), (g[v].cost, r[v].cost)):
             assert np.array_equal(np.asarray(a).view(np.uint32), np.asarray(b).view(np.uint32))
         assert np.array_equal(gc[v], rc[v])
+
+
+def test_source_less_view_and_repeated_source():
+    """A view without sources (its planes cost 2 and nothing is adopted) and a view listing one
+    source twice, in one batch with a normal view: a random-init run, a geometric iteration and
+    the read-out equal the oracle's bit for bit."""
+    import oracle
+
+    rig = scenes.oblique_rig(3, baseline=6.0)
+    sc = scenes.make_terrain_scene(rig, 160, 96, focal=500.0 * 160 / 640.0, seed=6)
+    pmp, fl = abi.fast_profile()
+    inp = scenes.kernel_input(sc, [[], [0, 0, 2], [0, 1]], [(20.0, 80.0)] * 3, pmp, fl)
+    views = [0, 1, 2]
+
+    def run(engine):
+        st = [abi.ViewState(160, 96) for _ in views]
+        engine.run(views, st, abi.run_params(0, False, True, 2, 1, False))
+        snap = abi.DepthSnapshot(3, 0, [x.depth.copy() for x in st])
+        engine.run(views, st, abi.run_params(0, True, False, 1, 2, False), snap)
+        return st, engine.filter(views, st, snap)
+
+    be = pm.HipPatchMatch()
+    with be.begin(inp) as s:
+        got, gc = run(s)
+    be.close()
+    ref, rc = run(oracle.Oracle(inp, threads=16))
+    for a, b in zip(got, ref):
+        for x, y in ((a.depth, b.depth), (a.normal, b.normal), (a.cost, b.cost)):
+            assert np.array_equal(np.asarray(x).view(np.uint32), np.asarray(y).view(np.uint32))
+    for a, b in zip(gc, rc):
+        assert np.array_equal(a, b)
+    assert np.all(gc[0] == 0)  # nothing to agree with
