@@ -56,3 +56,25 @@ def test_terrain_fusion_bit_exact():
     with pytest.raises(pm.PmError) as e:
         fusion.fuse(be, views, prm, 10)
     assert e.value.status == abi.C3D_INVALID_ARGUMENT
+
+
+def test_fusion_without_valid_depth():
+    """Every depth invalid (zero, negative or NaN): no point, on the device as in the oracle."""
+    import oracle
+    import test_fusion_oracle as t
+
+    sc, depth, normal = t._plane_scene()
+    src = scenes.nearest_sources(sc, [4] * len(sc.views))
+    bad = []
+    for i, d in enumerate(depth):
+        z = np.zeros_like(d)
+        z[::3] = -1.0
+        z[1::3] = np.nan
+        bad.append(z)
+    keep = []
+    views = fusion.fusion_views(sc, bad, normal, [np.full_like(d, 0.3) for d in depth], src, src, keep)
+    prm = fusion.FusionParams(min_consistent_views=2)
+    g = fusion.fuse(pm.HipPatchMatch(), views, prm, 10 ** 5)
+    r = oracle.fuse_consistency(views, prm, 10 ** 5)
+    assert len(r.positions) == 0
+    _same(g, r)

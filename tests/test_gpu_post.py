@@ -123,3 +123,28 @@ def test_densify_post_bit_exact(graph):
     changed = sum(int((_bits(go[v].depth) != _bits(p_out[v].depth)).sum()) for v in range(3))
     assert 0 < changed <= tm.post_removed + tm.post_filled
     be.close()
+
+
+@pytest.mark.parametrize("kind", ["all_invalid", "one_component"])
+def test_degenerate_maps(be, kind):
+    """A map with no valid pixel and a map that is one constant component: both filters equal the
+    oracle (nothing to erase or fill from in the first, one component larger than any speckle and
+    no hole in the second)."""
+    import oracle
+
+    h, w = 90, 130
+    rng = np.random.default_rng(31)
+    d = np.zeros((h, w), np.float32) if kind == "all_invalid" else np.full((h, w), 17.5, np.float32)
+    n = np.zeros((h, w, 3), np.float32)
+    n[..., 2] = -1.0
+    cost = rng.uniform(0.0, 1.5, (h, w)).astype(np.float32)
+    raw = rng.uniform(0.0, 1.5, (h, w)).astype(np.float32)
+    gray = rng.integers(0, 256, (h, w)).astype(np.uint8)
+    k = [0.9 * w, 0.9 * w, (w - 1) / 2.0, (h - 1) / 2.0]
+    p = abi.PostParams(fill_holes=True)
+    a = [d.copy(), n.copy(), cost.copy()]
+    b = [d.copy(), n.copy(), cost.copy()]
+    assert be.remove_speckles(a[0], a[1], a[2], p) == oracle.remove_speckles(b[0], b[1], b[2], p) == 0
+    assert be.fill_holes(a[0], a[1], a[2], raw, gray, k, p) == oracle.fill_holes(b[0], b[1], b[2], raw, gray, k, p) == 0
+    for x, y in zip(a, b):
+        assert np.array_equal(_bits(x), _bits(y))
