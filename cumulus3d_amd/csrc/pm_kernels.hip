@@ -60,7 +60,7 @@ __device__ __forceinline__ float pair_cost_h(const KParams& P, const RefWinT<WR,
                                              int y, float4 h, float& geo) {
     float H[9];
     float c = 2.0f;
-    if (homography(R, lp, ifx, ify, (float)x, (float)y, h.w, xyz(h), H)) c = ncc_cost_any<WR, WSTEP, PIPE>(P, rw, ref, S, H, x, y);
+    if (homography(R, lp, ifx, ify, (float)x, (float)y, h.w, xyz(h), H)) c = ncc_cost_any<WR, WSTEP, PIPE, GEO>(P, rw, ref, S, H, x, y);
     geo = GEO ? fminf(geo_error(P, R, S, pr, x, y, h.w), P.gmax) : 0.f;
     return c;
 }
