@@ -115,7 +115,124 @@ def load_library(path: str):
         for n in ("pmo_run", "pmo_evaluate", "pmo_filter", "pmo_upsample", "pmo_median", "pmo_densify", "pmo_run_rows",
                   "pmo_remove_speckles", "pmo_fill_holes", "pmo_pair_stats", "pmo_depth_ranges"):
             getattr(L, n).restype = C.c_int
+        ip = C.POINTER(C.c_int)
+        L.pmo_region_pick.argtypes = [fp, C.c_int, C.c_int, C.c_int, C.c_int, C.c_int, ip, ip]
+        L.pmo_region_pick.restype = C.c_int
+        L.pmo_transfer_plane.argtypes = [fp, fp, fp, fp]
+        L.pmo_transfer_plane.restype = C.c_int
+        L.pmo_select_views.argtypes = [C.POINTER(SelectParams), C.c_int, C.c_int, fp, fp, fp, fp, fp, fp, ip, ip]
+        L.pmo_select_views.restype = None
+        L.pmo_refine_accept.argtypes = [C.POINTER(SelectParams), C.c_int, C.c_int, fp, C.c_float, fp, fp, fp]
+        L.pmo_refine_accept.restype = C.c_int
+        L.pmo_perturb_normal_det.argtypes = [fp, C.c_float, fp, fp, C.c_float, fp]
+        L.pmo_perturb_normal_det.restype = None
+        L.pmo_topk_mean.argtypes = [fp, C.c_int, C.c_int]
+        L.pmo_topk_mean.restype = C.c_float
+        L.pmo_readout_source.argtypes = [C.POINTER(ReadoutParams), fp, C.c_float, fp, fp, C.c_int, C.c_float, C.c_float]
+        L.pmo_readout_source.restype = C.c_int
+        L.pmo_count_need.argtypes = [C.c_uint32, C.c_uint32]
+        L.pmo_count_need.restype = C.c_uint32
         return L
+
+
+class SelectParams(C.Structure):
+    """pmo_select_params: view-selection constants of one half step (tau(t), tau1, 1/(2 beta^2),
+    prev_view_weight, lambda, n1, n2)."""
+    _fields_ = [("tau_good", C.c_float), ("tau_bad", C.c_float), ("inv2beta2", C.c_float), ("prev_w", C.c_float),
+                ("lam", C.c_float), ("n1", C.c_int32), ("n2", C.c_int32)]
+
+
+class ReadoutParams(C.Structure):
+    """pmo_readout_params: read-out constants (cos of the minimum triangulation angle, emission
+    normalisation and 1/(2 sigma^2), q_min, maximum geometric error)."""
+    _fields_ = [("cos_filter_tri", C.c_float), ("ncc_norm", C.c_float), ("inv2ncc", C.c_float), ("q_min", C.c_float),
+                ("filter_gmax", C.c_float)]
+
+
+def _f32(a, n=None):
+    a = np.ascontiguousarray(a, np.float32)
+    return a, a.ctypes.data_as(C.POINTER(C.c_float))
+
+
+def region_pick(cost: np.ndarray, x: int, y: int, region: int):
+    """Neighbour chosen for propagation region `region` at (x, y): (qx, qy) or None."""
+    c, cp = _f32(cost)
+    h, w = c.shape
+    qx, qy = C.c_int(-1), C.c_int(-1)
+    ok = lib().pmo_region_pick(cp, w, h, x, y, region, C.byref(qx), C.byref(qy))
+    return (qx.value, qy.value) if ok else None
+
+
+def transfer_plane(q, ray_q, ray):
+    """The neighbour's plane q = (nx, ny, nz, d) moved onto `ray`, or None."""
+    q, qp = _f32(q)
+    rq, rqp = _f32(ray_q)
+    r, rp = _f32(ray)
+    out = np.zeros(4, np.float32)
+    ok = lib().pmo_transfer_plane(qp, rqp, rp, out.ctypes.data_as(C.POINTER(C.c_float)))
+    return out if ok else None
+
+
+def select_views(params: SelectParams, prior, m, g=None, vprev: int = 255):
+    """View selection over a cost table: prior (nsrc), m and g (nsrc, 9). Returns (wv, den, agg,
+    best hypothesis, best view)."""
+    pr, prp = _f32(prior)
+    n = pr.size
+    mm, mp = _f32(np.asarray(m, np.float32).reshape(n, 9))
+    gp = None
+    if g is not None:
+        gg, gp = _f32(np.asarray(g, np.float32).reshape(n, 9))
+    wv = np.zeros(max(n, 1), np.float32)
+    agg = np.zeros(9, np.float32)
+    den = C.c_float()
+    best, view = C.c_int(), C.c_int()
+    fp = C.POINTER(C.c_float)
+    lib().pmo_select_views(C.byref(params), n, vprev, prp, mp, gp, wv.ctypes.data_as(fp), C.cast(C.byref(den), fp),
+                           agg.ctypes.data_as(fp), C.byref(best), C.byref(view))
+    return wv[:n], float(den.value), agg, best.value, view.value
+
+
+def refine_accept(params: SelectParams, wv, den: float, m, g=None, bcost: float = 2.0):
+    """Refinement acceptance over candidate costs m (ncand, nsrc). Returns (taken index or -1,
+    final cost)."""
+    w, wp = _f32(wv)
+    n = w.size
+    mm = np.asarray(m, np.float32).reshape(-1, n)
+    mm, mp = _f32(mm)
+    gp = None
+    if g is not None:
+        gg, gp = _f32(np.asarray(g, np.float32).reshape(mm.shape))
+    b = C.c_float(bcost)
+    k = lib().pmo_refine_accept(C.byref(params), mm.shape[0], n, wp, den, mp, gp, C.cast(C.byref(b), C.POINTER(C.c_float)))
+    return k, float(b.value)
+
+
+def perturb_normal_det(a, u: float, n, ray, phi: float) -> np.ndarray:
+    """Normal perturbation given its draws (axis vector a, angle fraction u)."""
+    a, ap = _f32(a)
+    nn, np_ = _f32(n)
+    r, rp = _f32(ray)
+    out = np.zeros(3, np.float32)
+    lib().pmo_perturb_normal_det(ap, u, np_, rp, phi, out.ctypes.data_as(C.POINTER(C.c_float)))
+    return out
+
+
+def topk_mean(costs, k: int) -> float:
+    c, cp = _f32(costs if len(costs) else [0.0])
+    return float(lib().pmo_topk_mean(cp, len(costs), k))
+
+
+def readout_source(params: ReadoutParams, X, c, n, hok: bool, m: float, geo: float) -> bool:
+    """One source's read-out test at the 3-D point X (reference camera at the origin)."""
+    x, xp = _f32(X)
+    lx = float(np.sqrt(np.float32(x[0]) ** 2 + np.float32(x[1]) ** 2 + np.float32(x[2]) ** 2))
+    cc, cp = _f32(c)
+    nn, np_ = _f32(n)
+    return bool(lib().pmo_readout_source(C.byref(params), xp, lx, cp, np_, int(hok), m, geo))
+
+
+def count_need(min_num_consistent: int, nsrc: int) -> int:
+    return int(lib().pmo_count_need(min_num_consistent, nsrc))
 
 
 def rcp_sample(z: np.ndarray) -> np.ndarray:

@@ -717,10 +717,10 @@ static inline float pair_cost(const pmo_ctx* c, const launch_t* P, const rstats_
     return cc;
 }
 
-/* top-K mean (patchmatch.cu:326-345) */
+/* top-K mean (patchmatch.cu:326-345): the per-source costs in source order, then pmo_topk_mean */
 static float topk_cost(const pmo_ctx* c, const launch_t* P, const rstats_t* rs, const lvl_t* R, int view,
                        int nsrc, int x, int y, f4 h) {
-    float b0 = 1e30f, b1 = 1e30f, b2 = 1e30f;
+    float cs[MAXSRC];
     for (int j = 0; j < nsrc; ++j) {
         const pair_t* pr = &c->pairs[view * MAXSRC + j];
         const lvl_t* S = &c->lv[pr->src * c->nl + P->level];
@@ -728,15 +728,9 @@ static float topk_cost(const pmo_ctx* c, const launch_t* P, const rstats_t* rs, 
         float g = 0.f;
         float cc = pair_cost(c, P, rs, R, S, pr, lp, x, y, h, &g);
         if (P->geometric) cc = cc + c->lambda * g;
-        if (cc < b0) { b2 = b1; b1 = b0; b0 = cc; }
-        else if (cc < b1) { b2 = b1; b1 = cc; }
-        else if (cc < b2) { b2 = cc; }
+        cs[j] = cc;
     }
-    int k = c->topk < nsrc ? c->topk : nsrc;
-    float s = b0;
-    if (k >= 2) s += b1;
-    if (k >= 3) s += b2;
-    return k > 0 ? s / (float)k : 2.0f;
+    return pmo_topk_mean(cs, nsrc, c->topk);
 }
 
 /* uniform normal on the sphere facing the camera (patchmatch.cu:347-361) */
@@ -762,23 +756,14 @@ static float random_depth(uint64_t* st, float dmin, float dmax) {
     return 1.f / (a + pmo_rng_next(st) * (b - a));
 }
 
-/* rotate by u*phi about a random perpendicular axis, up to 4 halvings (patchmatch.cu:368-384) */
+/* rotate by u*phi about a random perpendicular axis, up to 4 halvings (patchmatch.cu:368-384); the
+ * draws are taken here in the device's order, the rotation is pmo_perturb_normal_det */
 static f3 perturb_normal(uint64_t* st, f3 n, f3 ray, float phi) {
     f3 a = random_normal(st, mk3(0.f, 0.f, 0.f));
-    f3 ax = cross3(a, n);
-    float l2 = dot3(ax, ax);
-    float ang = pmo_rng_next(st) * phi;
-    if (l2 < 1e-12f) return n;
-    ax = nrm3(ax);
-    f3 bn = cross3(ax, n);
-    for (int k = 0; k < 4; ++k) {
-        float sa, ca;
-        pmo_sincosf(ang, &sa, &ca);
-        f3 m = mk3(n.x * ca + bn.x * sa, n.y * ca + bn.y * sa, n.z * ca + bn.z * sa);
-        if (dot3(m, ray) < 0.f) return nrm3(m);
-        ang *= 0.5f;
-    }
-    return n;
+    float u = pmo_rng_next(st);
+    float av[3] = {a.x, a.y, a.z}, nv[3] = {n.x, n.y, n.z}, rv[3] = {ray.x, ray.y, ray.z}, o[3];
+    pmo_perturb_normal_det(av, u, nv, rv, phi, o);
+    return mk3(o[0], o[1], o[2]);
 }
 
 /* triangulation x incidence x resolution prior (patchmatch.cu:388-422) */
@@ -818,16 +803,288 @@ static float view_prior(const pmo_ctx* c, const lvl_t* R, const pair_t* pr, cons
     return tri * inc * res;
 }
 
-/* region offsets for the up direction (patchmatch.cu:424-436) */
-static const int V_OFF[7][2] = {{0, -1}, {-1, -2}, {1, -2}, {-2, -3}, {2, -3}, {-3, -4}, {3, -4}};
-static inline void rot_dir(int dir, int dx, int dy, int* ox, int* oy) {
-    *ox = dx;
-    *oy = dy;
-    for (int k = 0; k < dir; ++k) {
-        int t = *ox;
-        *ox = -*oy;
-        *oy = t;
+/* ================================================================ decision rules over tables
+ *
+ * The selection, acceptance and read-out rules of the half step, run start and read-out as pure
+ * functions of cost tables, so that tests/test_oracle_rules.py can pin each against cases derived
+ * by hand from the normative text (kernel.rs:50-66) and the kernel's read-out and top-K code. The
+ * per-pixel kernels below compute the tables and call these; the float operations and their order
+ * are the device's. */
+
+/* Region pick of the propagation step (kernel.rs:50-52, patchmatch.cu:480-498): among the
+ * region's offsets inside the image, the neighbour with the smallest stored cost, the first one in
+ * offset order on ties. Regions 0-3 are the V shapes up/right/down/left, 4-7 the bands. Returns 1
+ * with the neighbour in (*qx, *qy), 0 when no offset lies inside the image. */
+int pmo_region_pick(const float* cost, int w, int h, int x, int y, int region, int* qx_out, int* qy_out) {
+        /* region offsets for the up direction (patchmatch.cu:424-436) */
+    static const int V_OFF[7][2] = {{0, -1}, {-1, -2}, {1, -2}, {-2, -3}, {2, -3}, {-3, -4}, {3, -4}};
+    const int dir = region & 3;
+    const int strip = region >= 4;
+    const int cnt = strip ? 11 : 7;
+    float best = 1e30f;
+    int found = 0;
+    for (int k = 0; k < cnt; ++k) {
+        int dx = strip ? 0 : V_OFF[k][0], dy = strip ? -(3 + 2 * k) : V_OFF[k][1];
+        for (int r = 0; r < dir; ++r) { /* up -> right -> down -> left: (dx, dy) -> (-dy, dx) */
+            int t = dx;
+            dx = -dy;
+            dy = t;
+        }
+        const int qx = x + dx, qy = y + dy;
+        if (qx < 0 || qy < 0 || qx >= w || qy >= h) continue;
+        const float cq = cost[qy * w + qx];
+#if PMO_MUT == 19
+        if (cq <= best) {
+#else
+        if (cq < best) {
+#endif
+            best = cq;
+            *qx_out = qx;
+            *qy_out = qy;
+            found = 1;
+        }
     }
+    return found;
+}
+
+/* Moves the neighbour's plane q = (n, d_q) seen along ray_q onto the pixel's ray (kernel.rs:51-52,
+ * patchmatch.cu:499-510): d = d_q (n . ray_q) / (n . ray), kept when n . ray < -1e-6, d_q > 0 and
+ * d is positive and finite. Returns 1 with the moved plane in out. */
+int pmo_transfer_plane(const float q[4], const float ray_q[3], const float ray[3], float out[4]) {
+    const float den = q[0] * ray[0] + q[1] * ray[1] + q[2] * ray[2];
+    if (!(den < -1e-6f && q[3] > 0.f)) return 0;
+    const float d = q[3] * (q[0] * ray_q[0] + q[1] * ray_q[1] + q[2] * ray_q[2]) / den;
+    if (!(d > 0.f && isfinite(d))) return 0;
+    out[0] = q[0];
+    out[1] = q[1];
+    out[2] = q[2];
+    out[3] = d;
+    return 1;
+}
+
+/* View selection and aggregation (kernel.rs:53-58, patchmatch.cu:513-594) over the cost table of
+ * one pixel: prior[j] of each source, m[9 j + i] the photometric cost and g[9 j + i] the truncated
+ * geometric error of hypothesis i (0 = current, 1-8 = propagated) against source j (g NULL for a
+ * photometric run). Sources with prior <= 0 take no part (their m and g are not read). Writes the
+ * refinement weights wv[j] (> 0 for the sources that take part), their sum *den_out, the aggregated
+ * costs agg[9]; *best_out is the adopted hypothesis (the earlier index on ties) and *view_out the
+ * most important view (the earlier source on ties, 255 when no view is adopted). */
+void pmo_select_views(const pmo_select_params* p, int nsrc, int vprev, const float* prior, const float* m, const float* g,
+                      float* wv, float* den_out, float* agg, int* best_out, int* view_out) {
+    float num[9], numf[9];
+    for (int i = 0; i < 9; ++i) num[i] = numf[i] = 0.f;
+    float den = 0.f, wbest = 0.f, denf = 0.f;
+    int vbest = 255;
+    for (int j = 0; j < nsrc; ++j) {
+        wv[j] = 0.f;
+        if (!(prior[j] > 0.f)) continue;
+        const float* mj = m + 9 * j;
+        const float* gj = g ? g + 9 * j : NULL;
+        int good = 0, bad = 0;
+        float conf = 0.f;
+        for (int i = 1; i < 9; ++i) {
+            if (mj[i] < p->tau_good) {
+                good++;
+                conf += pmo_expf(-mj[i] * mj[i] * p->inv2beta2);
+            }
+            if (mj[i] > p->tau_bad) bad++;
+        }
+#if PMO_MUT == 11
+        int adopt = good >= p->n1 && bad < p->n2;
+#elif PMO_MUT == 12
+        int adopt = good > p->n1 && bad <= p->n2;
+#else
+        int adopt = good > p->n1 && bad < p->n2;
+#endif
+        int isprev = j == vprev;
+#if PMO_MUT == 13
+        float wp = adopt ? 1.f * conf / (float)good : (isprev ? p->prev_w : 0.f);
+#elif PMO_MUT == 14
+        float wp = adopt ? (isprev ? 2.f : 1.f) * conf / (float)good : 0.f;
+#else
+        float wp = adopt ? (isprev ? 2.f : 1.f) * conf / (float)good : (isprev ? p->prev_w : 0.f);
+#endif
+        float w2 = wp * prior[j];
+        wv[j] = w2 > 0.f ? w2 : -prior[j];
+        for (int i = 0; i < 9; ++i) numf[i] += prior[j] * (mj[i] + p->lambda * (gj ? gj[i] : 0.f));
+        denf += prior[j];
+        if (w2 > 0.f) {
+            for (int i = 0; i < 9; ++i) num[i] += w2 * (mj[i] + p->lambda * (gj ? gj[i] : 0.f));
+            den += w2;
+#if PMO_MUT == 16
+            if (w2 >= wbest) {
+#else
+            if (w2 > wbest) {
+#endif
+                wbest = w2;
+                vbest = j;
+            }
+        }
+    }
+    int bi9 = 0;
+    if (den > 0.f) {
+        for (int i = 0; i < 9; ++i) agg[i] = num[i] / den;
+        for (int i = 1; i < 9; ++i)
+#if PMO_MUT == 15
+            if (agg[i] <= agg[bi9]) bi9 = i;
+#else
+            if (agg[i] < agg[bi9]) bi9 = i;
+#endif
+    } else if (denf > 0.f) {
+        /* no adopted view: prior-weighted average, and refinement is compared with the priors */
+        for (int i = 0; i < 9; ++i) agg[i] = numf[i] / denf;
+        for (int i = 1; i < 9; ++i)
+            if (agg[i] < agg[bi9]) bi9 = i;
+#if PMO_MUT != 17
+        for (int j = 0; j < nsrc; ++j) wv[j] = -wv[j];
+#endif
+        den = denf;
+    } else {
+        for (int i = 0; i < 9; ++i) agg[i] = 2.f;
+    }
+    *den_out = den;
+    *best_out = bi9;
+    *view_out = vbest;
+}
+
+/* Refinement acceptance (kernel.rs:59-62, patchmatch.cu:608-653): candidate c's cost is the
+ * wv-weighted mean over the sources with wv[j] > 0 of m[c nsrc + j] + lambda g[c nsrc + j] (g NULL
+ * for a photometric run), divided by den, or 2 when den <= 0; candidates are tried in order and one
+ * is adopted when its cost is strictly below the running best *bcost (updated). Returns the index
+ * of the last adopted candidate, or -1. */
+int pmo_refine_accept(const pmo_select_params* p, int ncand, int nsrc, const float* wv, float den, const float* m, const float* g,
+                      float* bcost) {
+    int taken = -1;
+    for (int ci = 0; ci < ncand; ++ci) {
+        float cc;
+        if (den > 0.f) {
+            float nm = 0.f;
+            for (int j = 0; j < nsrc; ++j) {
+                if (!(wv[j] > 0.f)) continue;
+                nm += wv[j] * (m[ci * nsrc + j] + p->lambda * (g ? g[ci * nsrc + j] : 0.f));
+            }
+            cc = nm / den;
+        } else {
+            cc = 2.f;
+        }
+#if PMO_MUT == 20
+        if (cc <= *bcost) {
+#else
+        if (cc < *bcost) {
+#endif
+            *bcost = cc;
+            taken = ci;
+        }
+    }
+    return taken;
+}
+
+/* Normal perturbation given its random draws (kernel.rs:60-61, patchmatch.cu:368-384): `a` is the
+ * random unit vector that fixes the rotation axis a x n and u in (0, 1] the angle fraction. Rotates n
+ * by u phi about that axis toward b = axis x n; when the result does not face the camera (m . ray >= 0)
+ * the angle is halved, up to 4 tries in all, after which n is kept. n is also kept when a is
+ * parallel to n (|a x n|^2 < 1e-12). */
+void pmo_perturb_normal_det(const float a[3], float u, const float n[3], const float ray[3], float phi, float out[3]) {
+    f3 nn = mk3(n[0], n[1], n[2]);
+    f3 ax = cross3(mk3(a[0], a[1], a[2]), nn);
+    float l2 = dot3(ax, ax);
+    float ang = u * phi;
+    out[0] = n[0];
+    out[1] = n[1];
+    out[2] = n[2];
+    if (l2 < 1e-12f) return;
+    ax = nrm3(ax);
+    f3 bn = cross3(ax, nn);
+#if PMO_MUT == 18
+    for (int k = 0; k < 1; ++k) {
+#else
+    for (int k = 0; k < 4; ++k) {
+#endif
+        float sa, ca;
+        pmo_sincosf(ang, &sa, &ca);
+        f3 mm = mk3(nn.x * ca + bn.x * sa, nn.y * ca + bn.y * sa, nn.z * ca + bn.z * sa);
+        if (dot3(mm, mk3(ray[0], ray[1], ray[2])) < 0.f) {
+            mm = nrm3(mm);
+            out[0] = mm.x;
+            out[1] = mm.y;
+            out[2] = mm.z;
+            return;
+        }
+        ang *= 0.5f;
+    }
+}
+
+/* Top-K mean (kernel.rs:64-65 run start, patchmatch.cu:326-345): the mean of the min(k, n) smallest
+ * of the n costs, 2 when there is none. */
+float pmo_topk_mean(const float* cost, int n, int k) {
+    float b0 = 1e30f, b1 = 1e30f, b2 = 1e30f;
+    for (int j = 0; j < n; ++j) {
+        const float cc = cost[j];
+        if (cc < b0) { b2 = b1; b1 = b0; b0 = cc; }
+#if PMO_MUT == 21
+        else if (cc < b1) { b1 = cc; }
+#else
+        else if (cc < b1) { b2 = b1; b1 = cc; }
+#endif
+        else if (cc < b2) { b2 = cc; }
+    }
+#if PMO_MUT == 22
+    int kk = k;
+#else
+    int kk = k < n ? k : n;
+#endif
+    float s = b0;
+    if (kk >= 2) s += b1;
+    if (kk >= 3) s += b2;
+    return kk > 0 ? s / (float)kk : 2.0f;
+}
+
+/* One source's read-out test at a pixel (patchmatch.cu:714-733): X the pixel's 3-D point, lx = |X|,
+ * c the source centre and n the normal. The source counts when the triangulation angle at X is at
+ * least the minimum (cos <= cos_filter_tri), the surface faces the source (n . (c - X) > 0), the
+ * homography exists (hok), the visibility probability of the cost m, E/(E + 0.5) with
+ * E = ncc_norm exp(-m^2 inv2ncc), is not below q_min, and the geometric error is at most
+ * filter_gmax. Returns 1 when it counts. */
+static int readout_angles(const pmo_readout_params* k, const float X[3], float lx, const float c[3], const float n[3]) {
+    f3 Sv = mk3(c[0] - X[0], c[1] - X[1], c[2] - X[2]);
+    float ls = sqrtf(dot3(Sv, Sv));
+    if (!(ls > 0.f)) return 0;
+    float cost = -dot3(Sv, mk3(X[0], X[1], X[2])) / (ls * lx);
+#if PMO_MUT == 23
+    if (cost >= k->cos_filter_tri) return 0;
+#else
+    if (cost > k->cos_filter_tri) return 0;
+#endif
+    float cphi = dot3(Sv, mk3(n[0], n[1], n[2])) / ls;
+#if PMO_MUT == 24
+    if (cphi < 0.f) return 0;
+#else
+    if (cphi <= 0.f) return 0;
+#endif
+    return 1;
+}
+
+int pmo_readout_source(const pmo_readout_params* k, const float X[3], float lx, const float c[3], const float n[3], int hok, float m,
+                       float geo) {
+    if (!readout_angles(k, X, lx, c, n)) return 0;
+    if (!hok) return 0;
+    float e = k->ncc_norm * pmo_expf(-m * m * k->inv2ncc);
+    if (e / (e + 0.5f) < k->q_min) return 0;
+    if (geo > k->filter_gmax) return 0;
+    return 1;
+}
+
+/* Count threshold of the read-out (densify.rs:446-454): max(1, min(min_num_consistent, nsrc)),
+ * at most 255 (the count is a u8). */
+uint32_t pmo_count_need(uint32_t min_num_consistent, uint32_t nsrc) {
+#if PMO_MUT == 25
+    uint32_t need = min_num_consistent;
+#else
+    uint32_t need = min_num_consistent < nsrc ? min_num_consistent : nsrc;
+#endif
+    if (need < 1) need = 1;
+    if (need > 255) need = 255;
+    return need;
 }
 
 /* ================================================================ kernels, per pixel */
@@ -840,62 +1097,40 @@ static void half_pixel(const pmo_ctx* c, const launch_t* P, int view, const vsta
     const int nsrc = (int)kv->num_sources;
     const int pix = y * w + x;
     f4* pl = vs->plane;
-    const float* co = vs->cost;
     const f3 ray = pixel_ray(R, x, y);
+    const float rayv[3] = {ray.x, ray.y, ray.z};
 
+    /* 1. candidates: the current plane and one per region (patchmatch.cu:480-511) */
     f4 hyp[9];
     hyp[0] = pl[pix];
     for (int r = 0; r < 8; ++r) {
-        int dir = r & 3;
-        int strip = r >= 4;
-        int cnt = strip ? 11 : 7;
-        float best = 1e30f;
-        int bq = -1, bx = 0, by = 0;
-        for (int k = 0; k < cnt; ++k) {
-            int dx, dy;
-            if (strip) rot_dir(dir, 0, -(3 + 2 * k), &dx, &dy);
-            else rot_dir(dir, V_OFF[k][0], V_OFF[k][1], &dx, &dy);
-            int qx = x + dx, qy = y + dy;
-            if (qx < 0 || qy < 0 || qx >= w || qy >= h) continue;
-            float cq = co[qy * w + qx];
-            if (cq < best) {
-                best = cq;
-                bq = qy * w + qx;
-                bx = qx;
-                by = qy;
-            }
-        }
         hyp[1 + r] = hyp[0];
-        if (bq >= 0) {
-            f4 q = pl[bq];
-            f3 n = xyz(q);
-            f3 rq = pixel_ray(R, bx, by);
-            float den = dot3(n, ray);
-            if (den < -1e-6f && q.w > 0.f) {
-                float d = q.w * dot3(n, rq) / den;
-                if (d > 0.f && isfinite(d)) hyp[1 + r] = mk4(n, d);
-            }
+        int bx, by;
+        if (pmo_region_pick(vs->cost, w, h, x, y, r, &bx, &by)) {
+            const f4 q = pl[by * w + bx];
+            const f3 rq = pixel_ray(R, bx, by);
+            const float qv[4] = {q.x, q.y, q.z, q.w}, rqv[3] = {rq.x, rq.y, rq.z};
+            float o[4];
+            if (pmo_transfer_plane(qv, rqv, rayv, o)) hyp[1 + r] = (f4){o[0], o[1], o[2], o[3]};
         }
     }
 
+    /* 2. cost table over the sources, then view selection and aggregation (patchmatch.cu:513-594) */
     rstats_t rs;
     ref_stats(c, R, x, y, &rs);
     const int vprev = vs->bview[pix];
-    float num[9], numf[9], wv[MAXSRC];
-    for (int i = 0; i < 9; ++i) num[i] = numf[i] = 0.f;
-    float den = 0.f, wbest = 0.f, denf = 0.f;
-    int vbest = 255;
+    float prior[MAXSRC], mt[MAXSRC * 9], gt[MAXSRC * 9], wv[MAXSRC];
     for (int j = 0; j < nsrc; ++j) {
         const pair_t* pr = &c->pairs[view * MAXSRC + j];
         const lvl_t* S = &c->lv[pr->src * c->nl + P->level];
         const lpair_t* lp = &c->lpairs[((size_t)view * c->nl + P->level) * MAXSRC + j];
-        float m[9], gg[9], prior;
+        float* m = mt + 9 * j;
+        float* gg = gt + 9 * j;
         {
             float H[9];
             int ok = homography(R, lp, (float)x, (float)y, hyp[0].w, xyz(hyp[0]), H);
-            prior = view_prior(c, R, pr, H, ok, x, y, hyp[0]);
-            wv[j] = 0.f;
-            if (!(prior > 0.f)) continue;
+            prior[j] = view_prior(c, R, pr, H, ok, x, y, hyp[0]);
+            if (!(prior[j] > 0.f)) continue; /* not evaluated */
             m[0] = ok ? ncc_cost(c, &rs, S, H, x, y) : 2.f;
             gg[0] = P->geometric ? fminf(geo_error(c, R, S, pr, x, y, hyp[0].w), c->gmax) : 0.f;
         }
@@ -905,50 +1140,16 @@ static void half_pixel(const pmo_ctx* c, const launch_t* P, int view, const vsta
             m[i] = ok ? ncc_cost(c, &rs, S, H, x, y) : 2.f;
             gg[i] = P->geometric ? fminf(geo_error(c, R, S, pr, x, y, hyp[i].w), c->gmax) : 0.f;
         }
-        int good = 0, bad = 0;
-        float conf = 0.f;
-        for (int i = 1; i < 9; ++i) {
-            if (m[i] < P->tau_good) {
-                good++;
-                conf += pmo_expf(-m[i] * m[i] * c->inv2beta2);
-            }
-            if (m[i] > c->tau_bad) bad++;
-        }
-        int adopt = good > c->n1 && bad < c->n2;
-        int isprev = j == vprev;
-        float wp = adopt ? (isprev ? 2.f : 1.f) * conf / (float)good : (isprev ? c->prev_w : 0.f);
-        float w2 = wp * prior;
-        wv[j] = w2 > 0.f ? w2 : -prior;
-        for (int i = 0; i < 9; ++i) numf[i] += prior * (m[i] + c->lambda * gg[i]);
-        denf += prior;
-        if (w2 > 0.f) {
-            for (int i = 0; i < 9; ++i) num[i] += w2 * (m[i] + c->lambda * gg[i]);
-            den += w2;
-            if (w2 > wbest) {
-                wbest = w2;
-                vbest = j;
-            }
-        }
     }
-    float agg[9];
-    int bi9 = 0;
-    if (den > 0.f) {
-        for (int i = 0; i < 9; ++i) agg[i] = num[i] / den;
-        for (int i = 1; i < 9; ++i)
-            if (agg[i] < agg[bi9]) bi9 = i;
-    } else if (denf > 0.f) {
-        for (int i = 0; i < 9; ++i) agg[i] = numf[i] / denf;
-        for (int i = 1; i < 9; ++i)
-            if (agg[i] < agg[bi9]) bi9 = i;
-        for (int j = 0; j < nsrc; ++j) wv[j] = -wv[j];
-        den = denf;
-    } else {
-        for (int i = 0; i < 9; ++i) agg[i] = 2.f;
-    }
+    const pmo_select_params sp = {P->tau_good, c->tau_bad, c->inv2beta2, c->prev_w, c->lambda, c->n1, c->n2};
+    float den, agg[9];
+    int bi9, vbest;
+    pmo_select_views(&sp, nsrc, vprev, prior, mt, P->geometric ? gt : NULL, wv, &den, agg, &bi9, &vbest);
     f4 best = hyp[bi9];
     float bcost = agg[bi9];
 
-    /* refinement (patchmatch.cu:596-628) */
+    /* 3. refinement (patchmatch.cu:596-628): the six candidates from the pixel's random stream,
+     * their costs over the sources with wv > 0, then acceptance */
     uint64_t st = pmo_rng_state(c->in->seed, kv->key, (uint32_t)P->level, (uint32_t)P->run_id, (uint32_t)P->t,
                                 (uint32_t)P->color, (uint64_t)pix);
     float d_r = random_depth(&st, kv->depth_min, kv->depth_max);
@@ -959,55 +1160,27 @@ static void half_pixel(const pmo_ctx* c, const launch_t* P, int view, const vsta
     if (!(best.w > 0.f)) best.w = d_r;
     f3 n0 = xyz(best);
     f3 n_p = perturb_normal(&st, n0, ray, P->phi);
-    f4 cand[6] = {mk4(n0, d_r), mk4(n_r, best.w), mk4(n_r, d_r), mk4(n0, d_p), mk4(n_p, best.w), mk4(n_p, d_p)};
-    for (int ci = 0; ci < 6; ++ci) {
-        float cc;
-        if (den > 0.f) {
-            float nm = 0.f;
+    f4 cand[7] = {mk4(n0, d_r), mk4(n_r, best.w), mk4(n_r, d_r), mk4(n0, d_p), mk4(n_p, best.w), mk4(n_p, d_p)};
+    int ncand = 6;
+    /* 4. coarse-scale candidate at weakly textured pixels (patchmatch.cu:629-653) */
+    if (P->use_prior && rs.var < c->weak_var255) {
+        f4 q = vs->prior[pix];
+        if (q.w > 0.f && dot3(xyz(q), ray) < 0.f) cand[ncand++] = q;
+    }
+    float mc[7 * MAXSRC], gc[7 * MAXSRC];
+    if (den > 0.f)
+        for (int ci = 0; ci < ncand; ++ci)
             for (int j = 0; j < nsrc; ++j) {
                 if (!(wv[j] > 0.f)) continue;
                 const pair_t* pr = &c->pairs[view * MAXSRC + j];
                 const lvl_t* S = &c->lv[pr->src * c->nl + P->level];
                 const lpair_t* lp = &c->lpairs[((size_t)view * c->nl + P->level) * MAXSRC + j];
                 float g = 0.f;
-                float m = pair_cost(c, P, &rs, R, S, pr, lp, x, y, cand[ci], &g);
-                nm += wv[j] * (m + c->lambda * g);
+                mc[ci * nsrc + j] = pair_cost(c, P, &rs, R, S, pr, lp, x, y, cand[ci], &g);
+                gc[ci * nsrc + j] = g;
             }
-            cc = nm / den;
-        } else {
-            cc = 2.f;
-        }
-        if (cc < bcost) {
-            bcost = cc;
-            best = cand[ci];
-        }
-    }
-    /* coarse-scale candidate at weakly textured pixels (patchmatch.cu:629-653) */
-    if (P->use_prior && rs.var < c->weak_var255) {
-        f4 q = vs->prior[pix];
-        if (q.w > 0.f && dot3(xyz(q), ray) < 0.f) {
-            float cc;
-            if (den > 0.f) {
-                float nm = 0.f;
-                for (int j = 0; j < nsrc; ++j) {
-                    if (!(wv[j] > 0.f)) continue;
-                    const pair_t* pr = &c->pairs[view * MAXSRC + j];
-                    const lvl_t* S = &c->lv[pr->src * c->nl + P->level];
-                    const lpair_t* lp = &c->lpairs[((size_t)view * c->nl + P->level) * MAXSRC + j];
-                    float g = 0.f;
-                    float m = pair_cost(c, P, &rs, R, S, pr, lp, x, y, q, &g);
-                    nm += wv[j] * (m + c->lambda * g);
-                }
-                cc = nm / den;
-            } else {
-                cc = 2.f;
-            }
-            if (cc < bcost) {
-                bcost = cc;
-                best = q;
-            }
-        }
-    }
+    int taken = pmo_refine_accept(&sp, ncand, nsrc, wv, den, mc, P->geometric ? gc : NULL, &bcost);
+    if (taken >= 0) best = cand[taken];
     pl[pix] = best;
     vs->cost[pix] = bcost;
     vs->bview[pix] = (uint8_t)vbest;
@@ -1039,7 +1212,8 @@ static void eval_pixel(const pmo_ctx* c, const launch_t* P, int view, const vsta
     if (mode == 1) vs->bview[pix] = 255;
 }
 
-/* final read-out count at one pixel (patchmatch.cu:694-737) */
+/* final read-out count at one pixel (patchmatch.cu:694-737): each source's test is
+ * pmo_readout_source */
 static uint8_t filter_pixel(const pmo_ctx* c, int level, int view, const f4* plane, int w, int x, int y) {
     const lvl_t* R = &c->lv[view * c->nl + level];
     const c3d_kernel_view* kv = &c->in->views[view];
@@ -1050,26 +1224,21 @@ static uint8_t filter_pixel(const pmo_ctx* c, int level, int view, const f4* pla
         rstats_t rs;
         ref_stats(c, R, x, y, &rs);
         f3 ray = pixel_ray(R, x, y);
-        f3 X = mk3(ray.x * h.w, ray.y * h.w, h.w);
-        float lx = sqrtf(dot3(X, X));
+        const float X[3] = {ray.x * h.w, ray.y * h.w, h.w}, nv[3] = {h.x, h.y, h.z};
+        float lx = sqrtf(X[0] * X[0] + X[1] * X[1] + X[2] * X[2]);
+        const pmo_readout_params k = {c->cos_filter_tri, c->ncc_norm, c->inv2ncc, c->q_min, c->filter_gmax};
         for (int j = 0; j < (int)kv->num_sources; ++j) {
             const pair_t* pr = &c->pairs[view * MAXSRC + j];
             const lvl_t* S = &c->lv[pr->src * c->nl + level];
             const lpair_t* lp = &c->lpairs[((size_t)view * c->nl + level) * MAXSRC + j];
-            f3 Sv = mk3(pr->c[0] - X.x, pr->c[1] - X.y, pr->c[2] - X.z);
-            float ls = sqrtf(dot3(Sv, Sv));
-            if (!(ls > 0.f)) continue;
-            float cost = -dot3(Sv, X) / (ls * lx);
-            if (cost > c->cos_filter_tri) continue;
-            float cphi = dot3(Sv, xyz(h)) / ls;
-            if (cphi <= 0.f) continue;
+            /* the photometric and geometric terms only where the angle tests pass (they decide
+             * first); pmo_readout_source repeats those tests */
+            if (!readout_angles(&k, X, lx, pr->c, nv)) continue;
             float H[9];
-            if (!homography(R, lp, (float)x, (float)y, h.w, xyz(h), H)) continue;
-            float m = ncc_cost(c, &rs, S, H, x, y);
-            float e = c->ncc_norm * pmo_expf(-m * m * c->inv2ncc);
-            if (e / (e + 0.5f) < c->q_min) continue;
-            if (geo_error(c, R, S, pr, x, y, h.w) > c->filter_gmax) continue;
-            count++;
+            int hok = homography(R, lp, (float)x, (float)y, h.w, xyz(h), H);
+            float m = hok ? ncc_cost(c, &rs, S, H, x, y) : 2.f;
+            float ge = hok ? geo_error(c, R, S, pr, x, y, h.w) : 0.f;
+            count += pmo_readout_source(&k, X, lx, pr->c, nv, hok, m, ge);
         }
     }
     return (uint8_t)(count < 255 ? count : 255);
@@ -1595,10 +1764,7 @@ int pmo_densify(pmo_ctx* c, const uint32_t* views, uint32_t n, const c3d_schedul
     }
     for (uint32_t i = 0; !rc && i < n; ++i) {
         const c3d_kernel_view* kv = &c->in->views[views[i]];
-        uint32_t need = c->in->filter.min_num_consistent < kv->num_sources ? c->in->filter.min_num_consistent
-                                                                           : kv->num_sources;
-        if (need < 1) need = 1;
-        if (need > 255) need = 255;
+        const uint32_t need = pmo_count_need(c->in->filter.min_num_consistent, kv->num_sources);
         size_t np = (size_t)st[i].width * st[i].height;
         if (raw_depth && raw_depth[i]) memcpy(raw_depth[i], st[i].depth, np * sizeof(float));
         for (size_t q = 0; q < np; ++q) {

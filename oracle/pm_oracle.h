@@ -71,6 +71,28 @@ uint32_t pmo_num_levels(uint32_t min_side, uint32_t max_levels, uint32_t min_siz
 void pmo_build_pyramid(const uint8_t* gray, uint32_t w, uint32_t h, const double k[4],
                        uint32_t levels, uint8_t* const* out, float* k_out);
 
+/* decision rules of the half step, run start and read-out over tables (pm_oracle.c); the
+ * per-pixel kernels call these with the costs they compute */
+typedef struct pmo_select_params {
+    float tau_good, tau_bad, inv2beta2, prev_w, lambda;
+    int32_t n1, n2;
+} pmo_select_params;
+typedef struct pmo_readout_params {
+    float cos_filter_tri, ncc_norm, inv2ncc, q_min, filter_gmax;
+} pmo_readout_params;
+int pmo_region_pick(const float* cost, int w, int h, int x, int y, int region, int* qx, int* qy);
+int pmo_transfer_plane(const float q[4], const float ray_q[3], const float ray[3], float out[4]);
+void pmo_select_views(const pmo_select_params* p, int nsrc, int vprev, const float* prior, const float* m,
+                      const float* g, float* wv, float* den, float* agg, int* best, int* view);
+int pmo_refine_accept(const pmo_select_params* p, int ncand, int nsrc, const float* wv, float den,
+                      const float* m, const float* g, float* bcost);
+void pmo_perturb_normal_det(const float a[3], float u, const float n[3], const float ray[3], float phi,
+                            float out[3]);
+float pmo_topk_mean(const float* cost, int n, int k);
+int pmo_readout_source(const pmo_readout_params* k, const float X[3], float lx, const float c[3],
+                       const float n[3], int hok, float m, float geo);
+uint32_t pmo_count_need(uint32_t min_num_consistent, uint32_t nsrc);
+
 /* context over a kernel input (borrowed; must outlive the context) */
 pmo_ctx* pmo_create(const c3d_kernel_input* input, int num_threads);
 void pmo_destroy(pmo_ctx* ctx);

@@ -91,3 +91,78 @@ def test_sweep_closed_forms():
     # (row 0 and column 4 project onto the image border, where rounding may fall just outside)
     assert np.all(f[1:, 5:, kk] < 1e-9)
     assert f[1:, 5:, (kk + 8) % 32].mean() > 0.1
+
+
+def _torch_net(p, feat):
+    """The U-Net written independently with torch.nn.functional (NCHW, float64): 3x3 convolutions
+    with zero padding, 2x2 average pooling, nearest upsampling, the upsampled decoder input first
+    and the encoder skip second, a 1x1 head. Parameters in the flat layout of include/c3d_init.h:
+    weights [cout][k*k][cin] (tap t = (dy + r) k + (dx + r)), then bias, layer by layer."""
+    import torch
+    import torch.nn.functional as F
+
+    lay, _ = nno.layout()
+    pt = torch.tensor(p, dtype=torch.float64, requires_grad=True)
+
+    def conv(i, x, relu=True):
+        name, woff, boff, cin, cout, k, r = lay[i]
+        w = pt[woff:woff + cout * k * k * cin].reshape(cout, k, k, cin).permute(0, 3, 1, 2)
+        y = F.conv2d(x, w, pt[boff:boff + cout], padding=k // 2)
+        return F.relu(y) if r else y
+
+    x = torch.tensor(feat, dtype=torch.float64).permute(0, 3, 1, 2)
+    e0 = conv(1, conv(0, x))
+    e1 = conv(3, conv(2, F.avg_pool2d(e0, 2)))
+    e2 = conv(5, conv(4, F.avg_pool2d(e1, 2)))
+    e3 = conv(7, conv(6, F.avg_pool2d(e2, 2)))
+    up = lambda t: F.interpolate(t, scale_factor=2, mode="nearest")  # noqa: E731
+    d2 = conv(8, torch.cat([up(e3), e2], 1))
+    d1 = conv(9, torch.cat([up(d2), e1], 1))
+    d0 = conv(10, torch.cat([up(d1), e0], 1))
+    return conv(11, d0).permute(0, 2, 3, 1), pt
+
+
+def test_forward_matches_torch_functional():
+    """The NumPy forward pass (float64) equals the same network in torch.nn.functional: pins the
+    weight layout, the padding, the pooling, the upsampling and the order of the skip
+    concatenations, which the finite-difference test (forward against itself) cannot."""
+    p, feat, _, _ = _problem(seed=3, h=16, w=16)
+    A = nno.Net(p, bf=False).forward(feat)
+    out, _ = _torch_net(p, feat)
+    assert np.allclose(A["out"], out.detach().numpy(), rtol=1e-9, atol=1e-10)
+    # the comparison can fail: layer d1 reading its input channels in another order (its skip
+    # concatenated first) is told apart
+    q = p.copy()
+    lay, _ = nno.layout()
+    name, woff, boff, cin, cout, k, r = lay[9]
+    w = q[woff:woff + cout * k * k * cin].reshape(cout, k * k, cin)
+    w[:] = np.concatenate([w[:, :, 128:], w[:, :, :128]], axis=2)
+    assert not np.allclose(nno.Net(q, bf=False).forward(feat)["out"], out.detach().numpy(), rtol=1e-6)
+
+
+def test_backward_matches_torch_autograd():
+    """The analytic parameter gradient equals torch autograd through the torch network and a torch
+    statement of the loss (L1 on range-normalised inverse depth, 1 - cosine of the normal, binary
+    cross-entropy of the confidence against |1/idp - d| <= tol d, over labelled valid pixels)."""
+    import torch
+
+    p, feat, label, rng_ = _problem(seed=4)
+    _, net, A, g = _loss(p, feat, label, rng_)
+    grad = net.backward(A, g)
+    out, pt = _torch_net(p, feat)
+    lab = torch.tensor(label, dtype=torch.float64)
+    ok = lab[..., 0] > 0
+    ia, ib = 1.0 / rng_[0, 1], 1.0 / rng_[0, 0]
+    idp = ia + torch.sigmoid(out[..., 0]) * (ib - ia)
+    d_lab = torch.where(ok, lab[..., 0], torch.ones_like(lab[..., 0]))
+    ld = (idp - 1.0 / d_lab).abs() / (ib - ia)
+    v = out[..., 1:4]
+    nn_ = v / torch.sqrt((v * v).sum(-1, keepdim=True) + 1e-6)
+    ln = 1.0 - (nn_ * lab[..., 1:4]).sum(-1)
+    c = torch.sigmoid(out[..., 4])
+    tgt = ((1.0 / idp - lab[..., 0]).abs() <= 0.01 * lab[..., 0]).double()
+    lc = -(tgt * torch.log(c.clamp_min(1e-7)) + (1 - tgt) * torch.log((1 - c).clamp_min(1e-7)))
+    loss = ld[ok].sum() + WN * ln[ok].sum() + WC * lc[ok].sum()
+    loss.backward()
+    tg = pt.grad.numpy()
+    assert np.allclose(grad, tg, rtol=1e-7, atol=1e-10), float(np.abs(grad - tg).max())
