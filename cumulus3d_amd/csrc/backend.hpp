@@ -15,9 +15,18 @@ struct c3d_pm_backend {
     c3d_pm_options opts{};
     std::mutex lock;          // one PatchStereo session at a time (patchmatch.rs:96-97)
     hipStream_t stream = nullptr;
+    // Lifetime: every open session and network holds the backend. c3d_pm_destroy frees it at once
+    // when nothing holds it, otherwise it marks it and the last holder to close frees it.
+    std::mutex life;
+    int holders = 0;
+    bool destroy_pending = false;
 };
 
 namespace c3d {
 // Records `msg` as the calling thread's last error (c3d_pm_last_error) and returns `code`.
 int set_error(int code, const std::string& msg);
+// Takes a hold on `be` for a new session or network; fails when c3d_pm_destroy was already called.
+int backend_hold(c3d_pm_backend* be);
+// Drops a hold; frees the backend when it was the last one and c3d_pm_destroy was called.
+void backend_release(c3d_pm_backend* be);
 }  // namespace c3d

@@ -258,6 +258,8 @@ int c3d_init_create(c3d_pm_backend* be, uint32_t max_batch, uint32_t height, uin
     if (max_batch < 1 || height < 8 || width < 8 || height % 8 || width % 8)
         return set_error(C3D_INVALID_ARGUMENT, "batch >= 1 and height, width positive multiples of 8 required");
     NN_TRY(hipSetDevice(be->device));
+    const int hold = c3d::backend_hold(be);
+    if (hold) return hold;
     c3d_init_net* n = new c3d_init_net();
     n->be = be;
     n->B = (int)max_batch;
@@ -377,10 +379,12 @@ int c3d_init_create(c3d_pm_backend* be, uint32_t max_batch, uint32_t height, uin
 
 void c3d_init_destroy(c3d_init_net* n) {
     if (!n) return;
-    (void)hipSetDevice(n->be->device);
-    (void)hipStreamSynchronize(n->be->stream);
+    c3d_pm_backend* be = n->be;
+    (void)hipSetDevice(be->device);
+    (void)hipStreamSynchronize(be->stream);
     for (void* p : n->allocs) (void)hipFree(p);
     delete n;
+    c3d::backend_release(be);
 }
 
 uint64_t c3d_init_num_params(const c3d_init_net* n) { return n ? (uint64_t)n->nparam : 0; }

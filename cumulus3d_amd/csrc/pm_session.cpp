@@ -95,10 +95,15 @@ LevelPair level_pair(const float kr[4], const float kj[4], const float r[9], con
     return L;
 }
 
+// A device buffer that grows on demand and is freed with its owner (not copyable).
 template <class T>
 struct DevBuf {
     T* p = nullptr;
     size_t n = 0;
+    DevBuf() = default;
+    DevBuf(const DevBuf&) = delete;
+    DevBuf& operator=(const DevBuf&) = delete;
+    ~DevBuf() { release(); }
     hipError_t reserve(size_t want) {
         if (want <= n && p) return hipSuccess;
         if (p) (void)hipFree(p);
@@ -117,8 +122,34 @@ struct DevBuf {
 
 }  // namespace
 
+namespace {
+
+void free_backend(c3d_pm_backend* be) {
+    (void)hipSetDevice(be->device);
+    if (be->stream) (void)hipStreamDestroy(be->stream);
+    delete be;
+}
+
+}  // namespace
+
 namespace c3d {
 int set_error(int code, const std::string& msg) { return fail(code, msg); }
+
+int backend_hold(c3d_pm_backend* be) {
+    std::lock_guard<std::mutex> g(be->life);
+    if (be->destroy_pending) return fail(C3D_INVALID_ARGUMENT, "the backend was destroyed");
+    ++be->holders;
+    return C3D_OK;
+}
+
+void backend_release(c3d_pm_backend* be) {
+    bool last;
+    {
+        std::lock_guard<std::mutex> g(be->life);
+        last = --be->holders == 0 && be->destroy_pending;
+    }
+    if (last) free_backend(be);
+}
 }  // namespace c3d
 
 struct c3d_pm_session {
@@ -308,10 +339,14 @@ void unpack(const std::vector<float4>& h, const std::vector<BatchItem>& items, c
 }
 
 // Snapshot upload, only when the id changes (patchmatch.rs:573-604).
-int ensure_snapshot(c3d_pm_session* s, const c3d_depth_snapshot* snap) {
+// The snapshot's maps are laid out with the sizes of snap->level and read with the sizes of the
+// run's level, so the two must be the same scale.
+int ensure_snapshot(c3d_pm_session* s, const c3d_depth_snapshot* snap, uint32_t level) {
     if (!snap) return fail(C3D_INVALID_ARGUMENT, "a geometric run needs a depth snapshot");
     if ((int)snap->level >= s->nl) return fail(C3D_INVALID_ARGUMENT, "snapshot level out of range");
-    if (s->snap_valid && s->snap_id == snap->id) return C3D_OK;
+    if (snap->level != level)
+        return fail(C3D_INVALID_ARGUMENT, "snapshot of level " + std::to_string(snap->level) + " for a run at level " + std::to_string(level));
+    if (s->snap_valid && s->snap_id == snap->id && s->snap_level == (int)snap->level) return C3D_OK;
     std::vector<long long> offs((size_t)s->nv, -1);
     size_t total = 0;
     for (int v = 0; v < s->nv && (uint32_t)v < snap->num_maps; ++v) {
@@ -465,11 +500,17 @@ int c3d_pm_create(int ordinal, const c3d_pm_options* opts, c3d_pm_backend** out)
 
 const char* c3d_pm_name(const c3d_pm_backend* be) { return be ? be->name.c_str() : ""; }
 
-void c3d_pm_destroy(c3d_pm_backend* be) {
-    if (!be) return;
-    (void)hipSetDevice(be->device);
-    if (be->stream) (void)hipStreamDestroy(be->stream);
-    delete be;
+int c3d_pm_destroy(c3d_pm_backend* be) {
+    if (!be) return 0;
+    {
+        std::lock_guard<std::mutex> g(be->life);
+        if (be->holders > 0) {
+            be->destroy_pending = true;
+            return be->holders;
+        }
+    }
+    free_backend(be);
+    return 0;
 }
 
 int c3d_pm_begin(c3d_pm_backend* be, const c3d_kernel_input* in, c3d_pm_session** out) {
@@ -504,6 +545,8 @@ int c3d_pm_begin(c3d_pm_backend* be, const c3d_kernel_input* in, c3d_pm_session*
                 return fail(C3D_INVALID_ARGUMENT, "image larger than (w + 3)(h + 3) <= 2^24 pixels");
         }
     }
+    rc = c3d::backend_hold(be);
+    if (rc) return rc;
     c3d_pm_session* s = new c3d_pm_session();
     s->be = be;
     s->ks = ks;
@@ -657,7 +700,9 @@ void c3d_pm_end(c3d_pm_session* s) {
     s->post = nullptr;
     s->d_batch_lv.release();
     s->d_snap_lv.release();
-    delete s;
+    c3d_pm_backend* be = s->be;
+    delete s;  // unlocks the backend's session mutex
+    c3d::backend_release(be);
 }
 
 int c3d_pm_run(c3d_pm_session* s, const uint32_t* views, uint32_t n, c3d_view_state* states, const c3d_run_params* p,
@@ -668,7 +713,7 @@ int c3d_pm_run(c3d_pm_session* s, const uint32_t* views, uint32_t n, c3d_view_st
     rc = check_views(s, p->level, views, n, states);
     if (rc) return rc;
     if (p->geometric) {
-        rc = ensure_snapshot(s, snap);
+        rc = ensure_snapshot(s, snap, p->level);
         if (rc) return rc;
     }
     hipStream_t st = s->be->stream;
@@ -719,7 +764,7 @@ int c3d_pm_evaluate(c3d_pm_session* s, uint32_t level, const uint32_t* views, ui
     rc = check_views(s, level, views, n, states);
     if (rc) return rc;
     if (geometric) {
-        rc = ensure_snapshot(s, snap);
+        rc = ensure_snapshot(s, snap, level);
         if (rc) return rc;
     }
     hipStream_t st = s->be->stream;
@@ -748,7 +793,7 @@ int c3d_pm_filter(c3d_pm_session* s, const uint32_t* views, uint32_t n, const c3
     if (!s || !snap || (n && (!views || !states || !counts))) return fail(C3D_INVALID_ARGUMENT, "null argument");
     int rc = set_device(s->be);
     if (rc) return rc;
-    rc = ensure_snapshot(s, snap);
+    rc = ensure_snapshot(s, snap, snap->level);
     if (rc) return rc;
     const uint32_t level = snap->level;
     rc = check_views(s, level, views, n, states);

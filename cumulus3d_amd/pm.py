@@ -61,7 +61,7 @@ def load() -> C.CDLL:
         L.c3d_pm_name.argtypes = [C.c_void_p]
         L.c3d_pm_name.restype = C.c_char_p
         L.c3d_pm_destroy.argtypes = [C.c_void_p]
-        L.c3d_pm_destroy.restype = None
+        L.c3d_pm_destroy.restype = C.c_int
         L.c3d_pm_is_available.argtypes = []
         L.c3d_pm_begin.argtypes = [C.c_void_p, C.POINTER(abi.c3d_kernel_input), C.POINTER(C.c_void_p)]
         L.c3d_pm_run.argtypes = [C.c_void_p, u32p, C.c_uint32, vsp, C.POINTER(abi.c3d_run_params), snp]

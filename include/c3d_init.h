@@ -72,9 +72,12 @@ typedef struct c3d_init_train_params {
 
 /* Allocates the network on the device of `backend` for batches of up to max_batch images of
  * height x width (multiples of 8), with weights drawn from a He-normal distribution seeded by
- * `seed`. */
+ * `seed`. The network holds the backend until c3d_init_destroy: a c3d_pm_destroy in between
+ * defers the backend's release to that call. */
 int c3d_init_create(c3d_pm_backend* backend, uint32_t max_batch, uint32_t height, uint32_t width, uint64_t seed,
                     c3d_init_net** out);
+/* Frees the network and drops its hold on the backend (freeing the backend if c3d_pm_destroy
+ * was called while the network was open). */
 void c3d_init_destroy(c3d_init_net* net);
 /* Number of fp32 parameters (weights then bias of each layer, in layer order; weights [cout][k*k][cin]). */
 uint64_t c3d_init_num_params(const c3d_init_net* net);

@@ -186,8 +186,13 @@ int c3d_pm_create(int ordinal, const c3d_pm_options* opts, c3d_pm_backend** out)
  * patchmatch.rs:286-288). Valid until c3d_pm_destroy. */
 const char* c3d_pm_name(const c3d_pm_backend* backend);
 
-/* Releases the backend. Every session must have been ended. */
-void c3d_pm_destroy(c3d_pm_backend* backend);
+/* Releases the backend (drop of CudaPatchMatch, patchmatch.rs:285-298). Open sessions and
+ * learned-initialiser networks (c3d_init.h) hold the backend, as a Session borrows its backend for
+ * its lifetime in the reference (kernel.rs:244). Returns 0 when the backend was freed now; otherwise
+ * the number of sessions and networks still open: the backend is then freed when the last of them
+ * is closed (c3d_pm_end, c3d_init_destroy), and every further c3d_pm_begin or c3d_init_create on
+ * it fails with C3D_INVALID_ARGUMENT. The handle must not be used by the caller after this call. */
+int c3d_pm_destroy(c3d_pm_backend* backend);
 
 /* 1 when a gfx950 device can be opened; is_available (crates/cuda/src/device.rs:80-89). */
 int c3d_pm_is_available(void);
