@@ -3,6 +3,7 @@
 * libc3d_pm.so        the gfx950 PatchStereo engine and its learned initialiser (HIP kernels + C ABI of
                       include/c3d_pm.h and include/c3d_init.h)
 * libc3d_scenes.so    the synthetic terrain renderer used by tests and benchmarks (host C)
+* libc3d_render.so    the same renderer on the device (HIP), for the large benchmark workloads
 
 All HIP code is compiled for gfx950 only, with -ffp-contract=off (every multiply-add is written out)
 and without packed FP32 instructions.
@@ -64,13 +65,21 @@ def build_scenes() -> str:
     return out
 
 
+def build_render() -> str:
+    os.makedirs(LIB, exist_ok=True)
+    out = os.path.join(LIB, "libc3d_render.so")
+    _run([HIPCC, *HIP_FLAGS, "-shared", os.path.join(CSRC, "terrain.hip"), "-o", out + ".tmp"])
+    os.replace(out + ".tmp", out)
+    return out
+
+
 def build_all() -> list[str]:
     os.makedirs(LIB, exist_ok=True)
     stale = os.path.join(LIB, "libc3d_pm_repro.so")
     if os.path.exists(stale):
         os.remove(stale)
-    with ThreadPoolExecutor(2) as ex:
-        futs = [ex.submit(_engine, "c3d_pm", []), ex.submit(build_scenes)]
+    with ThreadPoolExecutor(3) as ex:
+        futs = [ex.submit(_engine, "c3d_pm", []), ex.submit(build_scenes), ex.submit(build_render)]
         return [f.result() for f in futs]
 
 

@@ -401,9 +401,11 @@ def make_terrain_scene(
     extent: float = 200.0,
     relief: float = 15.0,
     names: Optional[Sequence[str]] = None,
+    device: bool = False,
 ) -> DenseScene:
     """Renders seeded fBm terrain (extent x extent m, +-relief m) with random albedo from pinhole
-    cameras given as (center, target) pairs; images quantised to u8 after optional noise."""
+    cameras given as (center, target) pairs; images quantised to u8 after optional noise. With
+    `device`, the same scene model is rendered on the GPU (render.py)."""
     p = _TerrainParams(seed=seed, extent=extent, relief=relief, base_wavelength=60.0, octaves=6, albedo_scale=1.0, sky=0.7)
     cx, cy = (width - 1.0) / 2.0, (height - 1.0) / 2.0
     cams = (_TerrainCamera * len(cameras))()
@@ -421,7 +423,12 @@ def make_terrain_scene(
     dep = np.zeros((n, height, width), np.float32)
     nor = np.zeros((n, height, width, 3), np.float32)
     fp = C.POINTER(C.c_float)
-    _terrain_lib().c3d_render_terrain(C.byref(p), cams, n, img.ctypes.data_as(fp), dep.ctypes.data_as(fp), nor.ctypes.data_as(fp))
+    if device:
+        from . import render
+
+        render.render_terrain_device(p, cams, n, img, dep, nor)
+    else:
+        _terrain_lib().c3d_render_terrain(C.byref(p), cams, n, img.ctypes.data_as(fp), dep.ctypes.data_as(fp), nor.ctypes.data_as(fp))
     views = []
     for i, (r, t) in enumerate(rts):
         nm = names[i] if names else f"terrain_{seed}_{i}.png"
@@ -490,11 +497,13 @@ def kernel_input(
 
 
 def formation_rig(positions: int, cams_per_position: int = 3, spacing: float = 8.0, altitude: float = 30.0,
-                  ground_dist: float = 30.0, yaw_spread_deg: float = 30.0, cols: Optional[int] = None):
+                  ground_dist: float = 30.0, yaw_spread_deg: float = 30.0, cols: Optional[int] = None,
+                  heading_deg: float = 0.0, offset: tuple = (0.0, 0.0)):
     """A formation flight over the terrain: `positions` stations on a serpentine grid (`cols` per
     row, `spacing` m apart) at `altitude` m, each carrying `cams_per_position` cameras that look
     45 deg down (ground_dist = altitude), fanned by +-yaw_spread_deg about the flight heading.
-    Returns (center, target) pairs, station-major."""
+    The whole formation is turned by `heading_deg` about the vertical through the origin and then
+    moved by `offset` (x, y) metres. Returns (center, target) pairs, station-major."""
     cols = cols or max(1, int(round(math.sqrt(positions))))
     rows = (positions + cols - 1) // cols
     cams = []
@@ -510,13 +519,19 @@ def formation_rig(positions: int, cams_per_position: int = 3, spacing: float = 8
             yaw = math.radians(off)
             fwd = np.array([math.sin(yaw), math.cos(yaw), 0.0])
             cams.append((center, center + fwd * ground_dist - np.array([0.0, 0.0, altitude])))
+    if heading_deg != 0.0 or offset[0] != 0.0 or offset[1] != 0.0:
+        h = math.radians(heading_deg)
+        rot = np.array([[math.cos(h), -math.sin(h), 0.0], [math.sin(h), math.cos(h), 0.0], [0.0, 0.0, 1.0]])
+        shift = np.array([offset[0], offset[1], 0.0])
+        cams = [(rot @ c + shift, rot @ t + shift) for c, t in cams]
     return cams
 
 
 def make_mixed_scene(cameras: Sequence[tuple], sizes: Sequence[tuple], focal_1080: float = 500.0, seed: int = 1,
-                     noise_sigma: float = 0.0, noise_seed: int = 2) -> DenseScene:
+                     noise_sigma: float = 0.0, noise_seed: int = 2, device: bool = False) -> DenseScene:
     """Terrain views of uneven resolution: camera i is rendered at sizes[i] = (w, h) with focal
-    focal_1080 * w / 1920 (the same field of view at every resolution)."""
+    focal_1080 * w / 1920 (the same field of view at every resolution); `device` renders on the
+    GPU."""
     views = [None] * len(cameras)
     gd, gn = [None] * len(cameras), [None] * len(cameras)
     for size in sorted(set(tuple(s) for s in sizes)):
@@ -524,7 +539,7 @@ def make_mixed_scene(cameras: Sequence[tuple], sizes: Sequence[tuple], focal_108
         w, h = size
         sub = make_terrain_scene([cameras[i] for i in idx], w, h, focal=focal_1080 * w / 1920.0, seed=seed,
                                  noise_sigma=noise_sigma, noise_seed=noise_seed,
-                                 names=[f"terrain_{seed}_{i}.png" for i in idx])
+                                 names=[f"terrain_{seed}_{i}.png" for i in idx], device=device)
         for j, i in enumerate(idx):
             views[i], gd[i], gn[i] = sub.views[j], sub.gt_depth[j], sub.gt_normal[j]
     return DenseScene(views, [], gd, gn)
