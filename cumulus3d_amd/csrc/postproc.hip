@@ -21,8 +21,10 @@
 #include <hip/hip_runtime.h>
 #include <hipcub/hipcub.hpp>
 
+#include <algorithm>
 #include <cmath>
 #include <cstring>
+#include <mutex>
 #include <string>
 #include <vector>
 
@@ -355,6 +357,23 @@ __global__ __launch_bounds__(256) void k_fill(const PItem* items, float4* plane,
     wave_count(done, filled + blockIdx.y);
 }
 
+// planes (nx, ny, nz, d) from separate depth and 3-float normal arrays, and back
+__global__ __launch_bounds__(256) void k_pack_planes(const float* depth, const float* normal, float4* plane, long long n) {
+    const long long i = (long long)blockIdx.x * 256 + threadIdx.x;
+    if (i < n) plane[i] = make_float4(normal[3 * i], normal[3 * i + 1], normal[3 * i + 2], depth[i]);
+}
+
+__global__ __launch_bounds__(256) void k_unpack_planes(const float4* plane, float* depth, float* normal, long long n) {
+    const long long i = (long long)blockIdx.x * 256 + threadIdx.x;
+    if (i < n) {
+        const float4 q = plane[i];
+        depth[i] = q.w;
+        normal[3 * i] = q.x;
+        normal[3 * i + 1] = q.y;
+        normal[3 * i + 2] = q.z;
+    }
+}
+
 template <class T>
 hipError_t grow(T*& p, size_t& have, size_t want) {
     if (want <= have && p) return hipSuccess;
@@ -588,4 +607,122 @@ extern "C" int c3d_fill_holes(c3d_pm_backend* be, float* depth, float* normal, f
     p.remove_speckles = 0;
     p.fill_holes = 1;
     return post_one(be, depth, normal, cost, raw_cost, gray, k, width, height, p, filled);
+}
+
+// ---------------------------------------------------------------- allocation-free gap filling
+//
+// c3d_fill_holes_noalloc is c3d_fill_holes with every device buffer it needs (the staged inputs,
+// the packed planes, the validity mask, the item record and the counters) inside `scratch`, a
+// fixed array of device memory that each GPU holds from the library's load until the process
+// ends. It calls no allocating routine, so callers that cannot allocate (code under HIP graph
+// capture, tight serving loops) can use it. Every call on one GPU shares that GPU's array:
+// calls that overlap on one GPU race. The per-device address of the array comes from
+// hipGetSymbolAddress on the first call on that device; host code uses only that address.
+namespace {
+
+constexpr size_t NA_MAX_PIXELS = (size_t)1 << 21;  // 2,097,152 pixels: one 1920 x 1080 view fits
+constexpr size_t NA_PLANE = 0;                                     // float4 per pixel
+constexpr size_t NA_DEPTH = NA_PLANE + 16 * NA_MAX_PIXELS;         // float per pixel
+constexpr size_t NA_NORMAL = NA_DEPTH + 4 * NA_MAX_PIXELS;         // 3 floats per pixel
+constexpr size_t NA_COST = NA_NORMAL + 12 * NA_MAX_PIXELS;         // float per pixel
+constexpr size_t NA_RAW = NA_COST + 4 * NA_MAX_PIXELS;             // float per pixel
+constexpr size_t NA_GRAY = NA_RAW + 4 * NA_MAX_PIXELS;             // byte per pixel
+constexpr size_t NA_MASK = NA_GRAY + NA_MAX_PIXELS;                // byte per pixel
+constexpr size_t NA_ITEM = NA_MASK + NA_MAX_PIXELS;                // one PItem
+constexpr size_t NA_COUNTERS = NA_ITEM + 256;                      // 3 counters (post_run's layout)
+constexpr size_t NA_BYTES = NA_COUNTERS + 256;                     // 88,081,408 bytes
+static_assert(sizeof(c3d::PItem) <= 256, "item record larger than its slot");
+static_assert(NA_ITEM % 256 == 0 && NA_DEPTH % 256 == 0 && NA_COST % 256 == 0, "slots stay 256-byte aligned");
+
+constexpr int NA_MAX_DEVICES = 64;
+std::mutex na_mutex;
+unsigned char* na_base[NA_MAX_DEVICES] = {};
+
+}  // namespace
+
+static __device__ __align__(256) unsigned char scratch[NA_BYTES];
+
+namespace {
+
+// The device address of `scratch` on `device` (current), looked up once per device.
+int na_scratch(int device, unsigned char** out) {
+    if (device < 0 || device >= NA_MAX_DEVICES) return set_error(C3D_INVALID_ARGUMENT, "device ordinal beyond 64");
+    std::lock_guard<std::mutex> g(na_mutex);
+    if (!na_base[device]) {
+        void* p = nullptr;
+        const hipError_t e = hipGetSymbolAddress(&p, HIP_SYMBOL(scratch));
+        if (e != hipSuccess || !p)
+            return set_error(C3D_DEVICE, std::string("HIP: hipGetSymbolAddress(scratch): ") + hipGetErrorString(e));
+        na_base[device] = static_cast<unsigned char*>(p);
+    }
+    *out = na_base[device];
+    return C3D_OK;
+}
+
+}  // namespace
+
+extern "C" int c3d_fill_holes_noalloc(c3d_pm_backend* be, float* depth, float* normal, float* cost, const float* raw_cost,
+                                      const uint8_t* gray, const double k[4], uint32_t width, uint32_t height,
+                                      const c3d_post_params* params, uint64_t* filled) {
+    if (!be || !depth || !normal || !cost || !raw_cost || !gray || !k || !params)
+        return set_error(C3D_INVALID_ARGUMENT, "null argument");
+    if (width < 1 || height < 1) return set_error(C3D_INVALID_ARGUMENT, "bad image size");
+    const size_t np = (size_t)width * height;
+    if (np > NA_MAX_PIXELS)
+        return set_error(C3D_INVALID_ARGUMENT, "c3d_fill_holes_noalloc takes at most 2^21 pixels per call (its device scratch is "
+                                               "fixed); use c3d_fill_holes for larger images");
+    c3d_post_params p = *params;
+    p.remove_speckles = 0;
+    p.fill_holes = 1;
+    int rc = post_check(p);
+    if (rc) return rc;
+    P_TRY(hipSetDevice(be->device));
+    unsigned char* base = nullptr;
+    rc = na_scratch(be->device, &base);
+    if (rc) return rc;
+    hipStream_t st = be->stream;
+    float4* d_plane = reinterpret_cast<float4*>(base + NA_PLANE);
+    float* d_depth = reinterpret_cast<float*>(base + NA_DEPTH);
+    float* d_normal = reinterpret_cast<float*>(base + NA_NORMAL);
+    float* d_cost = reinterpret_cast<float*>(base + NA_COST);
+    float* d_raw = reinterpret_cast<float*>(base + NA_RAW);
+    uint8_t* d_gray = base + NA_GRAY;
+    uint8_t* d_mask = base + NA_MASK;
+    PItem* d_item = reinterpret_cast<PItem*>(base + NA_ITEM);
+    unsigned long long* d_counters = reinterpret_cast<unsigned long long*>(base + NA_COUNTERS);
+    // the one view as post_run describes it (postproc.rs:146 walk length)
+    PItem it{};
+    it.off = 0;
+    it.goff = 0;
+    it.w = (int)width;
+    it.h = (int)height;
+    it.min_area = 0u;
+    const unsigned long long d2 = (unsigned long long)width * width + (unsigned long long)height * height;
+    const double md = std::round(std::sqrt((double)d2) * p.fill_max_frac);
+    it.maxd = (int)std::max(1.0, std::min(md, 131072.0));
+    std::memcpy(it.k, k, sizeof it.k);
+    const float sf = p.fill_edge_step * 255.0f;  // postproc.rs:150, as in post_run
+    const int step = sf != sf ? 0 : (sf >= 2147483647.0f ? 2147483647 : (sf <= -2147483648.0f ? (-2147483647 - 1) : (int)sf));
+    P_TRY(hipMemcpyAsync(d_depth, depth, np * sizeof(float), hipMemcpyHostToDevice, st));
+    P_TRY(hipMemcpyAsync(d_normal, normal, 3 * np * sizeof(float), hipMemcpyHostToDevice, st));
+    P_TRY(hipMemcpyAsync(d_cost, cost, np * sizeof(float), hipMemcpyHostToDevice, st));
+    P_TRY(hipMemcpyAsync(d_raw, raw_cost, np * sizeof(float), hipMemcpyHostToDevice, st));
+    P_TRY(hipMemcpyAsync(d_gray, gray, np, hipMemcpyHostToDevice, st));
+    P_TRY(hipMemcpyAsync(d_item, &it, sizeof it, hipMemcpyHostToDevice, st));
+    const unsigned blocks = (unsigned)((np + 255) / 256);
+    hipLaunchKernelGGL(k_pack_planes, dim3(blocks), dim3(256), 0, st, d_depth, d_normal, d_plane, (long long)np);
+    hipLaunchKernelGGL(k_clear, dim3(1), dim3(1024), 0, st, d_counters, 3);
+    hipLaunchKernelGGL(k_mask, dim3(blocks, 1), dim3(256), 0, st, d_item, d_plane, d_mask);
+    hipLaunchKernelGGL(k_fill, dim3(blocks, 1), dim3(256), 0, st, d_item, d_plane, d_cost, d_raw, d_gray, d_mask, step,
+                       p.fill_max_cost, d_counters + 2);
+    hipLaunchKernelGGL(k_unpack_planes, dim3(blocks), dim3(256), 0, st, d_plane, d_depth, d_normal, (long long)np);
+    P_TRY(hipGetLastError());
+    unsigned long long nfilled = 0;
+    P_TRY(hipMemcpyAsync(depth, d_depth, np * sizeof(float), hipMemcpyDeviceToHost, st));
+    P_TRY(hipMemcpyAsync(normal, d_normal, 3 * np * sizeof(float), hipMemcpyDeviceToHost, st));
+    P_TRY(hipMemcpyAsync(cost, d_cost, np * sizeof(float), hipMemcpyDeviceToHost, st));
+    P_TRY(hipMemcpyAsync(&nfilled, d_counters + 2, sizeof nfilled, hipMemcpyDeviceToHost, st));
+    P_TRY(hipStreamSynchronize(st));
+    if (filled) *filled = nfilled;
+    return C3D_OK;
 }

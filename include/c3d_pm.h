@@ -348,6 +348,16 @@ int c3d_fill_holes(c3d_pm_backend* backend, float* depth, float* normal, float* 
                    const uint8_t* gray, const double k[4], uint32_t width, uint32_t height,
                    const c3d_post_params* params, uint64_t* filled);
 
+/* c3d_fill_holes without any allocation. No counterpart in the reference: it is for callers that
+ * cannot allocate device memory (code under HIP graph capture, tight serving loops). Same
+ * arguments, rules and results as c3d_fill_holes, but every device buffer it needs lives in one
+ * fixed device array of 88,081,408 bytes that each GPU holds from the library's load until the
+ * process ends. Images above 2^21 pixels (2,097,152; one 1920 x 1080 view fits) are refused with
+ * C3D_INVALID_ARGUMENT. Calls that overlap on one GPU share that array and race: serialise them. */
+int c3d_fill_holes_noalloc(c3d_pm_backend* backend, float* depth, float* normal, float* cost, const float* raw_cost,
+                           const uint8_t* gray, const double k[4], uint32_t width, uint32_t height,
+                           const c3d_post_params* params, uint64_t* filled);
+
 /* ---------------------------------------------------------------- autotuner (no reference counterpart) */
 
 /* Tile autotuner: times the red-black half step with each compiled workgroup shape — (rows of
