@@ -510,113 +510,174 @@ __global__ __launch_bounds__(256) void k_pair_costs(KParams P, int view, int slo
     out[i] = c;
 }
 
-// ================================================================ joint bilateral upsampling (patchmatch.cu:739-801)
+// ================================================================ plane records in LDS
+//
+// Both scale-change kernels below transfer a neighbour's plane onto a pixel's own viewing ray.
+// The neighbour q (plane normal n_q, depth d_q on its ray r_q) holds the plane n_q . X = c_q with
+// c_q = d_q (n_q . r_q), so the depth it gives pixel p is c_q / (n_q . r_p) (patchmatch.cu:405-412
+// transfer, same f32 operations and order). A workgroup therefore stages, once, the records
+// (n_q, c_q) of every neighbour its pixels can reach; c_q is NaN for a missing neighbour (outside
+// the image, or d_q not > 0), which every later test rejects exactly as the explicit checks would.
+
+__device__ __forceinline__ float4 plane_record(float4 q, f3 ray_q) {
+    const f3 n = xyz(q);
+    const float c = q.w > 0.f ? q.w * dot3(n, ray_q) : __int_as_float(0x7fc00000);
+    return mk4(n, c);
+}
+
+// ================================================================ joint bilateral upsampling
+// Output rules of pm_upsample (patchmatch.cu:739-801, upsample.rs:104-175): every low-resolution
+// neighbour in the 5 x 5 block around the pixel's position at the lower scale whose transferred depth
+// is positive and finite contributes exp(-ds/(2 sigma_s^2) - (I_p - I_q)^2/(2 sigma_c^2)) in scan
+// order; with no weight the nearest such neighbour (first in scan order on ties) is taken; the
+// normal is normalised and turned towards the camera.
+//
+// A 64 x 4 pixel tile reaches the low-resolution columns [x0/2 - 2, x0/2 + 33] and rows
+// [y0/2 - 2, y0/2 + 3]: 36 x 6 plane records and gray levels are staged in LDS by the whole
+// workgroup, so each of the 25 taps of a pixel is two LDS reads instead of a float4 and a byte
+// gather from global memory with its own transfer arithmetic.
+constexpr int UP_CW = PIX_TW / 2 + 4;  // 36 staged low-resolution columns
+constexpr int UP_CH = PIX_TH / 2 + 4;  // 6 staged rows
+
 __global__ __launch_bounds__(256) void k_upsample(KParams P) {
+    __shared__ float4 rec[UP_CH * UP_CW];
+    __shared__ float lgray[UP_CH * UP_CW];
     BatchItem it;
     int px0, py0;
-    if (!locate_tile(P, xcd_tile(blockIdx.x, P.ntiles), PIX_TW, PIX_TH, it, px0, py0)) return;
-    const int x = px0 + (threadIdx.x & 63), y = py0 + (threadIdx.x >> 6);
-    if (x >= it.w || y >= it.h) return;
+    if (!locate_tile(P, xcd_tile(blockIdx.x, P.ntiles), PIX_TW, PIX_TH, it, px0, py0)) return;  // uniform per workgroup
     const LevelDesc R = P.lv[it.view * P.nlevels + P.level];
     const LevelDesc L = P.lv[it.view * P.nlevels + P.level - 1];
     const float4* low = P.prior + it.low_off;
+    const int qxa = px0 / 2 - 2, qya = py0 / 2 - 2;  // first staged low-resolution column and row
+    for (int k = threadIdx.x; k < UP_CH * UP_CW; k += 256) {
+        const int qx = qxa + k % UP_CW, qy = qya + k / UP_CW;
+        float4 r = make_float4(0.f, 0.f, 0.f, __int_as_float(0x7fc00000));
+        float g = 0.f;
+        if (qx >= 0 && qx < L.w && qy >= 0 && qy < L.h) {
+            r = plane_record(low[qy * L.w + qx], pixel_ray(L, qx, qy));
+            g = (float)P.gray[L.goff + (size_t)qy * L.w + qx] * (1.0f / 255.0f);
+        }
+        rec[k] = r;
+        lgray[k] = g;
+    }
+    __syncthreads();
+    const int x = px0 + (threadIdx.x & 63), y = py0 + (threadIdx.x >> 6);
+    if (x >= it.w || y >= it.h) return;
     const f3 rp = pixel_ray(R, x, y);
     const float ip = (float)P.gray[R.goff + (size_t)y * R.w + x] * (1.0f / 255.0f);
     const float px = ((float)x - 0.5f) * 0.5f, py = ((float)y - 0.5f) * 0.5f;
     const int qx0 = (int)roundf(px), qy0 = (int)roundf(py);
-    float sw = 0.f, sd = 0.f;
-    f3 sn = mk3(0.f, 0.f, 0.f);
+    float sw = 0.f, sd = 0.f, snx = 0.f, sny = 0.f, snz = 0.f;
     float nd = 1e30f, ndep = 0.f;
     f3 nn = mk3(0.f, 0.f, 0.f);
-    for (int qy = qy0 - 2; qy <= qy0 + 2; ++qy) {
-        if (qy < 0 || qy >= L.h) continue;
-        for (int qx = qx0 - 2; qx <= qx0 + 2; ++qx) {
-            if (qx < 0 || qx >= L.w) continue;
-            float4 q = low[qy * L.w + qx];
-            if (!(q.w > 0.f)) continue;
-            f3 nq = xyz(q);
-            float den = dot3(nq, rp);
-            if (!(den < -1e-6f)) continue;
-            float d = q.w * dot3(nq, pixel_ray(L, qx, qy)) / den;
-            if (!(d > 0.f) || !isfinite(d)) continue;
-            float ds = ((float)qx - px) * ((float)qx - px) + ((float)qy - py) * ((float)qy - py);
+#pragma unroll
+    for (int dy = -2; dy <= 2; ++dy) {
+        const int qy = qy0 + dy;
+        const int row = (qy - qya) * UP_CW - qxa;
+#pragma unroll
+        for (int dx = -2; dx <= 2; ++dx) {
+            const int qx = qx0 + dx;
+            const float4 r = rec[row + qx];
+            const f3 nq = xyz(r);
+            const float den = dot3(nq, rp);
+            const float d = r.w / den;
+            if (!(den < -1e-6f) || !(d > 0.f) || !isfinite(d)) continue;
+            const float ds = ((float)qx - px) * ((float)qx - px) + ((float)qy - py) * ((float)qy - py);
             if (ds < nd) {
                 nd = ds;
                 ndep = d;
                 nn = nq;
             }
-            float iq = (float)P.gray[L.goff + (size_t)qy * L.w + qx] * (1.0f / 255.0f);
-            float wgt = c3d_expf(-ds * P.up_inv2s - (ip - iq) * (ip - iq) * P.up_inv2c);
+            const float iq = lgray[row + qx];
+            const float wgt = c3d_expf(-ds * P.up_inv2s - (ip - iq) * (ip - iq) * P.up_inv2c);
             sw += wgt;
             sd += wgt * d;
-            sn.x += wgt * nq.x;
-            sn.y += wgt * nq.y;
-            sn.z += wgt * nq.z;
+            snx += wgt * nq.x;
+            sny += wgt * nq.y;
+            snz += wgt * nq.z;
         }
     }
     float4 o = make_float4(0.f, 0.f, 0.f, 0.f);
-    float d = 0.f;
-    f3 nv = mk3(0.f, 0.f, 0.f);
-    bool ok = false;
-    if (sw > 1e-12f) {
-        d = sd / sw;
-        nv = sn;
-        ok = true;
-    } else if (nd < 1e29f) {
-        d = ndep;
-        nv = nn;
-        ok = true;
-    }
-    if (ok && dot3(nv, nv) > 1e-24f) {
-        nv = nrm3(nv);
-        if (dot3(nv, rp) >= 0.f) nv = mk3(-nv.x, -nv.y, -nv.z);
-        o = mk4(nv, d);
+    const bool weighted = sw > 1e-12f;
+    if (weighted || nd < 1e29f) {
+        const float d = weighted ? sd / sw : ndep;
+        f3 nv = weighted ? mk3(snx, sny, snz) : nn;
+        if (dot3(nv, nv) > 1e-24f) {
+            nv = nrm3(nv);
+            if (dot3(nv, rp) >= 0.f) nv = mk3(-nv.x, -nv.y, -nv.z);
+            o = mk4(nv, d);
+        }
     }
     P.plane[it.off + y * it.w + x] = o;
 }
 
-// ================================================================ 5x5 median plane filter (patchmatch.cu:803-849)
+// ================================================================ 5 x 5 median plane filter
+// Output rule of pm_median (patchmatch.cu:803-849, upsample.rs:177-260): a pixel with a positive
+// depth takes, among the neighbours of its 5 x 5 block (clipped to the image) whose plane transfers
+// to a positive finite depth on its ray, the lower median of those depths, ties ordered by pixel
+// index; its output is that neighbour's normal with the transferred depth. Other pixels keep their
+// plane.
+//
+// The 64 x 4 tile and its 2-pixel halo (68 x 8 records) are staged in LDS. Each pixel then holds
+// its 25 candidate depths in registers (a rejected one as +inf) and picks the median by rank:
+// candidate k has rank #{j > k : d_j < d_k} + #{j < k : d_j <= d_k} (slot order is pixel-index
+// order, so this is the index tie-break), and the candidate of rank (count - 1) / 2 is the median.
+// No data-dependent branches or register indexing: every lane does the same 600 comparisons.
+constexpr int MED_CW = PIX_TW + 4;
+constexpr int MED_CH = PIX_TH + 4;
+
 __global__ __launch_bounds__(256) void k_median(KParams P) {
+    __shared__ float4 rec[MED_CH * MED_CW];
     BatchItem it;
     int px0, py0;
-    if (!locate_tile(P, xcd_tile(blockIdx.x, P.ntiles), PIX_TW, PIX_TH, it, px0, py0)) return;
-    const int x = px0 + (threadIdx.x & 63), y = py0 + (threadIdx.x >> 6);
-    if (x >= it.w || y >= it.h) return;
+    if (!locate_tile(P, xcd_tile(blockIdx.x, P.ntiles), PIX_TW, PIX_TH, it, px0, py0)) return;  // uniform per workgroup
     const LevelDesc R = P.lv[it.view * P.nlevels + P.level];
     const float4* in = P.prior + it.off;
     const int w = it.w, h = it.h;
-    float4 self = in[y * w + x];
+    const int xa = px0 - 2, ya = py0 - 2;
+    for (int k = threadIdx.x; k < MED_CH * MED_CW; k += 256) {
+        const int qx = xa + k % MED_CW, qy = ya + k / MED_CW;
+        float4 r = make_float4(0.f, 0.f, 0.f, __int_as_float(0x7fc00000));
+        if (qx >= 0 && qx < w && qy >= 0 && qy < h) r = plane_record(in[qy * w + qx], pixel_ray(R, qx, qy));
+        rec[k] = r;
+    }
+    __syncthreads();
+    const int x = px0 + (threadIdx.x & 63), y = py0 + (threadIdx.x >> 6);
+    if (x >= w || y >= h) return;
+    const float4 self = in[y * w + x];
     float4 out = self;
     if (self.w > 0.f) {
         const f3 rp = pixel_ray(R, x, y);
-        float ds[25];
-        int qs[25];
-        int c = 0;
-        int ya = y - 2 > 0 ? y - 2 : 0, yb = y + 3 < h ? y + 3 : h;
-        int xa = x - 2 > 0 ? x - 2 : 0, xb = x + 3 < w ? x + 3 : w;
-        for (int qy = ya; qy < yb; ++qy)
-            for (int qx = xa; qx < xb; ++qx) {
-                float4 q = in[qy * w + qx];
-                if (!(q.w > 0.f)) continue;
-                f3 nq = xyz(q);
-                float den = dot3(nq, rp);
-                if (!(den < -1e-6f)) continue;
-                float d = q.w * dot3(nq, pixel_ray(R, qx, qy)) / den;
-                if (!(d > 0.f) || !isfinite(d)) continue;
-                int qi = qy * w + qx;
-                int k = c++;
-                while (k > 0 && (ds[k - 1] > d || (ds[k - 1] == d && qs[k - 1] > qi))) {
-                    ds[k] = ds[k - 1];
-                    qs[k] = qs[k - 1];
-                    --k;
+        const int base = (y - ya - 2) * MED_CW + (x - xa - 2);
+        float d[25];
+        int cnt = 0;
+#pragma unroll
+        for (int k = 0; k < 25; ++k) {
+            const float4 r = rec[base + (k / 5) * MED_CW + k % 5];
+            const float den = dot3(xyz(r), rp);
+            const float t = r.w / den;
+            const bool ok = den < -1e-6f && t > 0.f && isfinite(t);
+            d[k] = ok ? t : __int_as_float(0x7f800000);
+            cnt += ok ? 1 : 0;
+        }
+        if (cnt > 0) {
+            const int m = (cnt - 1) / 2;
+            int pick = 0;
+#pragma unroll
+            for (int k = 0; k < 25; ++k) {
+                int rank = 0;
+#pragma unroll
+                for (int j = 0; j < 25; ++j) {
+                    if (j < k) rank += d[j] <= d[k] ? 1 : 0;
+                    if (j > k) rank += d[j] < d[k] ? 1 : 0;
                 }
-                ds[k] = d;
-                qs[k] = qi;
+                pick = (rank == m && d[k] < __int_as_float(0x7f800000)) ? k : pick;
             }
-        if (c > 0) {
-            int m = (c - 1) / 2;
-            float4 q = in[qs[m]];
-            out = make_float4(q.x, q.y, q.z, ds[m]);
+            float dm = d[0];
+#pragma unroll
+            for (int k = 1; k < 25; ++k) dm = pick == k ? d[k] : dm;
+            const float4 r = rec[base + (pick / 5) * MED_CW + pick % 5];
+            out = make_float4(r.x, r.y, r.z, dm);
         }
     }
     P.plane[it.off + y * w + x] = out;

@@ -527,6 +527,17 @@ int c3d_pm_begin(c3d_pm_backend* be, const c3d_kernel_input* in, c3d_pm_session*
     if (in->num_levels < 1) return fail(C3D_INVALID_ARGUMENT, "num_levels must be >= 1");
     int rc = set_device(be);
     if (rc) return rc;
+    // the hold refuses a destroyed backend before waiting for its session lock (an open session
+    // keeps that lock); a failed begin gives the hold back after unlocking
+    rc = c3d::backend_hold(be);
+    if (rc) return rc;
+    struct Hold {
+        c3d_pm_backend* be;
+        bool kept = false;
+        ~Hold() {
+            if (!kept) c3d::backend_release(be);
+        }
+    } hold{be};
     std::unique_lock<std::mutex> guard(be->lock);
     const int nv = (int)in->num_views, nl = (int)in->num_levels;
     // validate
@@ -545,10 +556,9 @@ int c3d_pm_begin(c3d_pm_backend* be, const c3d_kernel_input* in, c3d_pm_session*
                 return fail(C3D_INVALID_ARGUMENT, "image larger than (w + 3)(h + 3) <= 2^24 pixels");
         }
     }
-    rc = c3d::backend_hold(be);
-    if (rc) return rc;
     c3d_pm_session* s = new c3d_pm_session();
     s->be = be;
+    hold.kept = true;  // the session owns the hold from here; c3d_pm_end gives it back
     s->ks = ks;
     s->nv = nv;
     s->nl = nl;
