@@ -142,6 +142,6 @@ def compare(a: Cloud, b: Cloud, radius: float = 0.05, max_queries: int = 2_000_0
     rng = np.random.default_rng(0)
     qa = a.positions if len(a.positions) <= max_queries else a.positions[rng.choice(len(a.positions), max_queries, replace=False)]
     qb = b.positions if len(b.positions) <= max_queries else b.positions[rng.choice(len(b.positions), max_queries, replace=False)]
-    da, _ = cKDTree(b.positions.astype(np.float64)).query(qa.astype(np.float64), k=1, workers=-1)
-    db, _ = cKDTree(a.positions.astype(np.float64)).query(qb.astype(np.float64), k=1, workers=-1, distance_upper_bound=radius)
+    da, _ = cKDTree(b.positions.astype(np.float64)).query(qa.astype(np.float64), k=1, workers=16)
+    db, _ = cKDTree(a.positions.astype(np.float64)).query(qb.astype(np.float64), k=1, workers=16, distance_upper_bound=radius)
     return float(np.median(da)), float(np.mean(np.isfinite(db)))

@@ -195,28 +195,36 @@ def test_learned_densify_refines_like_the_oracle():
 
 def test_trained_network_gains_over_random_init():
     """The committed weights (cumulus3d_amd/weights/init_unet.npy, trained on plain-PatchStereo
-    labels) against the same short schedule from the random initialisation: on held-out terrain at
-    1920 x 1080, one refinement iteration and one geometric iteration after the network agree with
-    plain PatchStereo (7 iterations + 1 x 2 geometric) on more pixels than after a random start."""
+    labels of other terrains) earn their place on held-out terrain at 1920 x 1080: 4 formation
+    stations x 3 cameras, 4-8 sources per view, the zone workload's method with its geometric round.
+    With one refinement iteration and one geometric iteration after the network, the depth maps
+    agree with plain PatchStereo (7 iterations + 1 x 2 geometric) within 1 % on at least 99 % of
+    the plain maps' valid pixels (the accuracy gate); from the reference's random initialisation the
+    same schedule does not reach the gate."""
     import os
 
-    from cumulus3d_amd import autotune, train_init
+    from cumulus3d_amd import autotune
 
     wpath = os.path.join(os.path.dirname(os.path.dirname(os.path.abspath(__file__))), "cumulus3d_amd", "weights", "init_unet.npy")
-    sc = train_init.scene(90001, width=1920, height=1080, focal=500.0)
+    rng = np.random.default_rng(11)
+    cams = scenes.formation_rig(4, 3, spacing=8.0)
+    sc = scenes.make_mixed_scene(cams, [(1920, 1080)] * 12, seed=11, noise_sigma=0.01, noise_seed=12, device=True)
+    src = scenes.nearest_sources(sc, [int(k) for k in rng.integers(4, 9, 12)])
     pmp, fl = abi.fast_profile()
-    inp = scenes.kernel_input(sc, [[1, 2], [0, 2], [0, 1]], [(15.0, 120.0)] * 3, pmp, fl)
+    inp = scenes.kernel_input(sc, src, [(15.0, 120.0)] * 12, pmp, fl)
     be = pm.HipPatchMatch()
-    net = init_net.InitNet(be, 3, 272, 480)
+    net = init_net.InitNet(be, 8, 272, 480)
     net.set_params(np.load(wpath, allow_pickle=False).astype(np.float32))
-    views = [0, 1, 2]
+    views = list(range(12))
     with be.begin(inp) as s:
         p_outs, _, p_raw, p_tm = s.densify(views, abi.Schedule(photometric_iters=7, geometric_rounds=1, geometric_iters=2))
         _, _, r_raw, r_tm = s.densify(views, abi.Schedule(photometric_iters=1, geometric_rounds=1, geometric_iters=1))
-        _, _, l_raw, l_tm, _, _ = init_net.densify_learned(s, net, views, init_net.LearnedSchedule(1, 1, 1))
+        _, _, l_raw, l_tm, idep, _ = init_net.densify_learned(s, net, views, init_net.LearnedSchedule(1, 1, 1, conf_min=0.25),
+                                                              want_init=True)
+    net.close()
     learned = autotune.depth_agreement(p_outs, p_raw, l_raw)
     random = autotune.depth_agreement(p_outs, p_raw, r_raw)
-    print(f"agreement with plain within 1 %: learned {learned:.4f} ({l_tm.total_ms:.1f} ms), "
-          f"random {random:.4f} ({r_tm.total_ms:.1f} ms), plain {p_tm.total_ms:.1f} ms")
-    # measured on this scene: learned 0.9776, random 0.9728 at about the same time; plain takes 3.6x longer
-    assert learned > 0.97 and learned > random + 0.002
+    alone = autotune.depth_agreement(p_outs, p_raw, idep)
+    print(f"agreement with plain within 1 %: network alone {alone:.4f}; 1 + 1x1 after the network {learned:.4f} "
+          f"({l_tm.total_ms:.1f} ms), after a random start {random:.4f} ({r_tm.total_ms:.1f} ms); plain {p_tm.total_ms:.1f} ms")
+    assert learned >= 0.99 > random
