@@ -1,33 +1,47 @@
 """Throughput of the gfx950 PatchStereo engine on one MI355X.
 
-Workload (the reference hot path's single-GPU configuration): seeded fBm terrain (200 x 200 m,
-+-15 m relief, random albedo) seen by 7 pinhole cameras 30 m up at 45 deg oblique, f = 500 px,
-1920 x 1080 gray with Gaussian noise sigma = 0.01, quantised to u8. Depth maps for all 7 views,
-each view using the other 6 as sources (1 reference + 6 sources, as the geometric pass needs the
-sources' depth), window radius 5 step 2, depth range 15-120 m, FP32:
-7 photometric iterations from random initialisation, one geometric-consistency round of 2
-iterations, read-out filter and count threshold (crates/dense/src/densify.rs:350-454 single-level
-schedule), all on the device (c3d_pm_densify).
+Default workload (zone108, the configuration the metric is quoted on): the full zone of a
+formation flight, 36 stations x 3 cameras = 108 views at 1920 x 1080 over one seeded fBm terrain
+(200 x 200 m, +-15 m relief, random albedo; Gaussian noise sigma = 0.01, quantised to u8), each view
+with its 4-8 nearest overlapping views as sources (seeded), window radius 5 step 2, depth range
+15-120 m, FP32. Method: the learned initialiser (bf16 U-Net on the quarter-resolution image and a
+32-depth plane sweep of two sources) + batched PatchStereo (one launch per half step over all 108
+views) + a geometric-consistency round + read-out filter, all on the device
+(c3d_pm_densify_learned). The schedule after the network is picked by the autotuner on 12 of the
+views: the fastest candidate that keeps a geometric round and reaches the accuracy gate.
 
-A step = that whole job. Images are uploaded before the timed region; states stay in HBM.
-value = sum of final W*H over the 7 depth maps / step time (Mpx/s).
+A step = that whole job with images resident in HBM (uploaded before the timed region) and states
+kept on the device. value = sum of W*H over the 108 depth maps / step time (Mpx/s).
+
+Measured next to it, in the same run:
+  plain       plain PatchStereo on the same views: 7 photometric iterations from random init, one
+              geometric round of 2 iterations, read-out (c3d_pm_densify) - the accuracy reference
+              (<= 1 % relative depth error on >= 99 % of its valid pixels is the gate);
+  random      the reference's random initialisation on the learned arm's own schedule (the control
+              arm: what the network's start is worth), and the fastest random-init schedule that
+              reaches the gate (chosen by the same autotuner);
+  begin->download  one learned job timed from c3d_pm_begin (image upload, source tables) to the
+              downloaded read-out, as the metric's Mpx/s is defined on the host boundary;
+  fusion      agreement fusion (min views 3) of the learned and plain maps and the cloud comparison
+              (median point distance, completeness within 5 cm);
+  cpu_baseline  the CPU oracle (C, OpenMP) on a band of one view (the reference has no CPU
+              PatchStereo), scaled to the plain job.
 
 Other workloads (--workload):
+  zone7      7 views 1920x1080, each with the other 6 as sources, 7 photometric iterations + one
+             geometric round of 2, read-out (plain PatchStereo on one GPU, the single-reference
+             configuration with noise sigma = 0.01).
   batched48  48 views of a formation flight, resolutions drawn from 960x540 / 1280x720 /
              1920x1080, 2-8 sources each, one batch per half-step, the whole schedule replayed as a
              hipGraph.
-  zone108    108 views (36 stations x 3 cameras) at 1920x1080, 4-8 sources each: plain PatchStereo
-             (the default schedule above) against the learned initialiser (network on the quarter-
-             resolution image, 2 refinement iterations, one geometric round of 2), the accuracy of
-             the learned result against the plain one (<= 1 % relative depth error on >= 99 % of
-             the plain method's valid pixels) and the fused clouds of both (median point distance,
-             completeness within 5 cm). value = the learned method's Mpx/s.
+  fast960    the fast profile over 3 scales on 240 views of 960x540 (the reference's published
+             densification setting).
 """
 from __future__ import annotations
 
 import argparse
-import dataclasses
 import ctypes as C
+import dataclasses
 import json
 import os
 import platform
@@ -41,14 +55,19 @@ import numpy as np  # noqa: E402
 
 from cumulus3d_amd import abi, pm, scenes  # noqa: E402
 
+METRIC = "depth-map Mpx/s at <=1% rel. depth error vs plain PatchStereo; HBM GB/s"
 HBM_PEAK_GBS = 8000.0
-# HBM bytes per half-step launch on the default workload (7 x 1920x1080, 6 sources, 7 + 1x2
-# iterations), averaged over its 18 launches: rocprofv3 FETCH_SIZE x 2.0 + WRITE_SIZE x 1.0 (factors
-# from a 512 MiB copy in the same run), profiles/k_half-zone7-7x1080p-6src.md. FETCH_SIZE counts L2
-# misses, which the Infinity Cache mostly serves: an upper bound on HBM reads.
-K_HALF_TRAFFIC_ZONE7 = 25.995e9
+# HBM bytes per half-step launch, from rocprofv3 --pmc FETCH_SIZE x 2.0 + WRITE_SIZE x 1.0 (the
+# factors from a 512 MiB device copy in the same run); FETCH_SIZE counts L2 misses, which the
+# Infinity Cache mostly serves: an upper bound on HBM reads.
+#   zone7: 7 x 1920x1080, 6 sources, 7 + 1x2 iterations, averaged over its 18 launches
+#          (profiles/k_half-zone7-7x1080p-6src.md)
+#   zone108_1_1x1: the 108-view zone's learned job, 1 refinement + 1x1 geometric iterations,
+#          averaged over its 4 launches (profiles/k_half-zone108-learned.md)
+K_HALF_TRAFFIC = {"zone7": 25.995e9, "zone108_1_1x1": 73.396e9}
 FP32_PEAK_TFLOPS = 157.3
 FLOP_PER_SAMPLE = 28  # projective transform 9, reciprocal 1, bilinear 7, scale 1, accumulations 6 (rounded up)
+WN_FAST = 6  # samples per window side at radius 5 step 2
 
 
 def log(msg):
@@ -80,9 +99,32 @@ def copy_calibration(mib):
     log(f"copy calibration: {mib} MiB device-to-device")
 
 
+def host_cpu():
+    """(threads this process may use, logical CPUs of the host, CPU model)."""
+    try:
+        avail = len(os.sched_getaffinity(0))
+    except AttributeError:
+        avail = os.cpu_count() or 1
+    omp = os.environ.get("OMP_NUM_THREADS")
+    threads = min(avail, int(omp)) if omp and omp.isdigit() and int(omp) > 0 else avail
+    model = platform.processor() or platform.machine()
+    try:
+        with open("/proc/cpuinfo") as f:
+            for line in f:
+                if line.startswith("model name"):
+                    model = line.split(":", 1)[1].strip()
+                    break
+    except OSError:
+        pass
+    return threads, os.cpu_count() or avail, model
+
+
+# ---------------------------------------------------------------- workloads
+
+
 def make_workload(width, height, nviews, noise, seed, window=(5, 2)):
     rig = scenes.oblique_rig(nviews, baseline=6.0)
-    sc = scenes.make_terrain_scene(rig, width, height, focal=500.0, seed=seed, noise_sigma=noise, noise_seed=seed + 1)
+    sc = scenes.make_terrain_scene(rig, width, height, focal=500.0, seed=seed, noise_sigma=noise, noise_seed=seed + 1, device=True)
     src = [[j for j in range(nviews) if j != v] for v in range(nviews)]
     pmp, fl = abi.fast_profile()
     pmp = abi.PmParams(**{**pmp.__dict__, "window_radius": window[0], "window_step": window[1]})
@@ -99,7 +141,7 @@ def make_batched_workload(nviews, noise, seed):
     rng = np.random.default_rng(seed)
     cams = scenes.formation_rig(nviews // 3, 3)
     sizes = [BATCH_SIZES[k] for k in rng.integers(0, 3, len(cams))]
-    sc = scenes.make_mixed_scene(cams, sizes, seed=seed, noise_sigma=noise, noise_seed=seed + 1)
+    sc = scenes.make_mixed_scene(cams, sizes, seed=seed, noise_sigma=noise, noise_seed=seed + 1, device=True)
     src = scenes.nearest_sources(sc, [int(k) for k in rng.integers(2, 9, len(cams))])
     pmp, fl = abi.fast_profile()
     inp = scenes.kernel_input(sc, src, [(15.0, 120.0)] * len(cams), pmp, fl)
@@ -112,7 +154,7 @@ def make_fast960_workload(nviews, noise, seed, levels_fn):
     960x540, each with its 10 nearest overlapping views as sources (NeighborParams::num_views),
     gray pyramids built on the device by `levels_fn`."""
     cams = scenes.formation_rig(nviews // 3, 3)
-    sc = scenes.make_mixed_scene(cams, [(960, 540)] * len(cams), seed=seed, noise_sigma=noise, noise_seed=seed + 1)
+    sc = scenes.make_mixed_scene(cams, [(960, 540)] * len(cams), seed=seed, noise_sigma=noise, noise_seed=seed + 1, device=True)
     src = scenes.nearest_sources(sc, [10] * len(cams))
     pmp, fl = abi.fast_profile()
     nl = pm.num_levels(540, 3, 64)
@@ -120,10 +162,53 @@ def make_fast960_workload(nviews, noise, seed, levels_fn):
     return sc, inp, src
 
 
+def make_zone_workload(noise, seed, stations=36, device=True, size=(1920, 1080)):
+    """The zone: `stations` formation stations x 3 cameras (108 views at 36) at 1920 x 1080, each
+    view with its 4-8 nearest overlapping views as sources (seeded)."""
+    rng = np.random.default_rng(seed)
+    cams = scenes.formation_rig(stations, 3, spacing=8.0)
+    sc = scenes.make_mixed_scene(cams, [size] * len(cams), seed=seed, noise_sigma=noise, noise_seed=seed + 1, device=device)
+    src = scenes.nearest_sources(sc, [int(k) for k in rng.integers(4, 9, len(cams))])
+    pmp, fl = abi.fast_profile()
+    inp = scenes.kernel_input(sc, src, [(15.0, 120.0)] * len(cams), pmp, fl)
+    return sc, inp, src
+
+
+# ---------------------------------------------------------------- roofline and CPU baseline
+
+
+def half_step_work(inp, src, launches):
+    """Average algorithmic bytes and FLOP per half-step launch over `launches` = [(level,
+    geometric?)]. Per pixel and half step: S = 15 nsrc WN^2 bilinear samples (upper bound: 9
+    propagated + 6 refinement hypotheses over every source), 28 FLOP each; compulsory HBM bytes
+    58 + 2 (1 + nsrc), plus 8 nsrc for geometric iterations. A launch updates npix/2 pixels of
+    every view."""
+    wn = (2 * inp.pm.window_radius) // inp.pm.window_step + 1
+    nv = len(src)
+    tot_b = tot_f = 0.0
+    for lv, geo in launches:
+        for v in range(nv):
+            h, w = inp.level_shape(v, lv)
+            ns = len(src[v])
+            tot_b += h * w / 2 * (58 + 2 * (1 + ns) + (8 * ns if geo else 0))
+            tot_f += FLOP_PER_SAMPLE * 15 * ns * wn * wn * h * w / 2
+    return tot_b / len(launches), tot_f / len(launches)
+
+
+def roofline_objects(kernel, avg_bytes, flop, half_ms, traffic):
+    achieved_gbs = avg_bytes / (half_ms / 1e3) / 1e9
+    achieved_tflops = flop / (half_ms / 1e3) / 1e12
+    return ({"bound": "hbm", "kernel": kernel, "achieved": round(achieved_gbs, 2), "peak": HBM_PEAK_GBS, "unit": "GB/s",
+             "frac": round(achieved_gbs / HBM_PEAK_GBS, 5), "traffic": traffic, "avg_launch_ms": round(half_ms, 3),
+             "algorithmic_bytes_per_launch": int(avg_bytes)},
+            {"bound": "valu-fp32", "kernel": kernel, "achieved": round(achieved_tflops, 2), "peak": FP32_PEAK_TFLOPS, "unit": "TFLOP/s",
+             "frac": round(achieved_tflops / FP32_PEAK_TFLOPS, 4), "algorithmic_flop_per_launch": int(flop)})
+
+
 def cpu_baseline(inp, rows, threads, photometric_iters, view=0):
     """The CPU oracle (C, OpenMP over rows) on a bounded sample: a band of `rows` rows of `view`,
     random-init photometric run (evaluation of the band and its 24-row reach, then the iterations
-    on the band only). Returns the rate in pixel-iterations per second (eval counted as one)."""
+    on the band only). Returns (pixel-iterations per second (eval counted as one), seconds, text)."""
     import oracle
 
     o = oracle.Oracle(inp, threads=threads)
@@ -138,116 +223,185 @@ def cpu_baseline(inp, rows, threads, photometric_iters, view=0):
     return px_iters / dt, dt, f"view {view}, rows {r0}..{r0 + rows} of {w}x{h}, random init + {photometric_iters} photometric iterations"
 
 
-def make_zone_workload(noise, seed, stations=36):
-    """The 108-view zone: 36 stations x 3 cameras at 1920 x 1080, 4-8 sources each (seeded)."""
-    rng = np.random.default_rng(seed)
-    cams = scenes.formation_rig(stations, 3, spacing=8.0)
-    sc = scenes.make_mixed_scene(cams, [(1920, 1080)] * len(cams), seed=seed, noise_sigma=noise, noise_seed=seed + 1)
-    src = scenes.nearest_sources(sc, [int(k) for k in rng.integers(4, 9, len(cams))])
-    pmp, fl = abi.fast_profile()
-    inp = scenes.kernel_input(sc, src, [(15.0, 120.0)] * len(cams), pmp, fl)
-    return sc, inp, src
+def cpu_baseline_object(inp, src, views, rows, photometric_iters, geometric_rounds, geometric_iters, job_mpx):
+    """cpu_baseline for a plain job over `views` (all at the top scale of a single-level input).
+    The sample's rate per pixel-iteration and source is scaled to the job's pixel-iterations x
+    sources (the cost of a pixel-iteration grows with its view's source count); a geometric
+    iteration is counted as a photometric one (it costs more: the figure is generous to the CPU)."""
+    threads, ncpu, model = host_cpu()
+    nsrc_med = sorted(len(src[v]) for v in views)[len(views) // 2]
+    view = min(views, key=lambda v: (abs(len(src[v]) - nsrc_med), -np.prod(inp.level_shape(v, 0)), v))
+    rate, dt, sample = cpu_baseline(inp, rows, threads, photometric_iters, view)
+    rate_src = rate * len(src[view])
+    per_px = photometric_iters + 1 + geometric_rounds * (geometric_iters + 1) + 1  # evals, iterations, read-out
+    job_work = sum(np.prod(inp.level_shape(v, 0)) * per_px * len(src[v]) for v in views)
+    cpu_s = job_work / rate_src
+    log(f"cpu baseline: {rate:.0f} px-iter/s with {len(src[view])} sources on {threads} threads ({dt:.1f} s)")
+    return {
+        "value": round(job_mpx / cpu_s, 5),
+        "unit": "Mpx/s",
+        "cores": threads,
+        "kind": "port",
+        "sample": f"{sample}, {len(src[view])} sources: {dt:.1f} s, {rate:.0f} px-iter/s; scaled per pixel-iteration and source "
+                  f"to the plain job ({job_work / 1e9:.2f} G px-iter-sources, {cpu_s:.0f} s). The reference has no CPU "
+                  f"PatchStereo: this is the C oracle (OpenMP, {threads} threads of {ncpu} logical CPUs, {model}).",
+        "host_logical_cpus": ncpu,
+        "cpu_model": model,
+    }
+
+
+# ---------------------------------------------------------------- zone (default)
+
+
+def agreement(plain_outs, plain_raw, raws, plain_normals=None, normals=None):
+    """Depth agreement (within 1 %) of `raws` with the plain raw depth on the plain method's valid
+    pixels (read-out depth > 0), and normal agreement (within 2 deg) when normals are given."""
+    ok_d = ok_n = tot = 0
+    for i, (o, dp, dl) in enumerate(zip(plain_outs, plain_raw, raws)):
+        m = o.depth > 0
+        tot += int(m.sum())
+        ok_d += int((np.abs(dl[m] - dp[m]) <= 0.01 * dp[m]).sum())
+        if normals is not None:
+            cosang = np.clip((plain_normals[i][m] * normals[i][m]).sum(-1), -1, 1)
+            ok_n += int((cosang >= np.cos(np.radians(2.0))).sum())
+    return ok_d / max(tot, 1), (ok_n / max(tot, 1) if normals is not None else None), tot
+
+
+def timed(name, run, steps, warmup):
+    """Runs `run()` warmup + steps times; returns (ms per step, timings of the timed steps)."""
+    for i in range(warmup):
+        run()
+    device_sync()
+    tms = []
+    t = time.perf_counter()
+    for i in range(steps):
+        tms.append(run())
+    device_sync()
+    ms = (time.perf_counter() - t) / max(steps, 1) * 1e3
+    log(f"{name}: {ms:.1f} ms per job ({steps} steps, {warmup} warmup)")
+    return ms, tms
+
+
+def confirm(rows, gate, run_full, name):
+    """The fastest calibration candidate (rows = [(candidate, agreement, ms)]) whose agreement on
+    every view of the job also reaches `gate`: candidates passing on the calibration views are
+    tried fastest first on the whole job. Returns (candidate, full agreement, run_full's output,
+    [(candidate, full agreement)] tried). When none passes, the most accurate one tried (or, when
+    none passed calibration, the most accurate calibration candidate)."""
+    tried = []
+    best = None
+    for cand, a, ms in sorted([r for r in rows if r[1] >= gate], key=lambda r: r[2]):
+        acc, out = run_full(cand)
+        tried.append((cand, acc))
+        log(f"{name} {cand}: {acc:.4f} on every view (calibration {a:.4f}, {ms:.0f} ms)")
+        if acc >= gate:
+            return cand, acc, out, tried
+        if best is None or acc > best[1]:
+            best = (cand, acc, out)
+    if best is None:
+        cand = max(rows, key=lambda r: r[1])[0]
+        acc, out = run_full(cand)
+        tried.append((cand, acc))
+        best = (cand, acc, out)
+    return best[0], best[1], best[2], tried
 
 
 def zone_main(args):
-    """Plain vs learned-init PatchStereo on the 108-view zone, with fusion and cloud comparison."""
-    from cumulus3d_amd import fusion, init_net
+    from cumulus3d_amd import autotune, fusion, init_net
 
     wpath = os.path.join(ROOT, "cumulus3d_amd", "weights", "init_unet.npy")
     if not os.path.exists(wpath):
         print(f"{wpath} is missing: train it with python -m cumulus3d_amd.train_init", file=sys.stderr)
         sys.exit(1)
     t0 = time.perf_counter()
-    sc, inp, src = make_zone_workload(0.01, seed=5, stations=(args.views or 108) // 3)
+    stations = (args.views or 108) // 3
+    sc, inp, src = make_zone_workload(0.01, seed=5, stations=stations)
     nviews = len(sc.views)
     views = list(range(nviews))
     job_mpx = sum(v.width * v.height for v in sc.views) / 1e6
-    log(f"zone: {nviews} views, {job_mpx:.1f} Mpx rendered in {time.perf_counter() - t0:.1f} s")
+    log(f"zone: {nviews} views, {job_mpx:.1f} Mpx rendered on the device in {time.perf_counter() - t0:.1f} s; "
+        f"sources {min(len(s) for s in src)}-{max(len(s) for s in src)}")
     be = pm.HipPatchMatch()
     net = init_net.InitNet(be, 8, 272, 480)
     net.set_params(np.load(wpath, allow_pickle=False).astype(np.float32))
-    # read-out post-processing of the fast profile (speckle removal + hole filling, densify.rs:455-462)
-    post = abi.PostParams.fast() if args.post else None
     plain_s = abi.Schedule(photometric_iters=args.photometric_iters, geometric_rounds=args.geometric_rounds,
-                           geometric_iters=args.geometric_iters, post=post)
-    learned_s = init_net.LearnedSchedule(refine_iters=args.refine_iters, geometric_rounds=args.geometric_rounds,
-                                         geometric_iters=args.geometric_iters, post=post)
-    res = {}
-    tune_rows = None
-    with be.begin(inp) as s:
-        if args.autotune:
-            from cumulus3d_amd import autotune
+                           geometric_iters=args.geometric_iters)
+    calib = views[: min(12, nviews)]
+    s = be.begin(inp)
+    shape = (4, 1)
+    if args.tile_autotune:
+        shape, shape_ms = s.autotune(calib, level=0)
+        log(f"tile autotuner on {len(calib)} views: (rows, lanes per pixel) {shape}; (2,1) {shape_ms[0]:.1f} ms, "
+            f"(4,1) {shape_ms[1]:.1f} ms, (4,2) {shape_ms[2]:.1f} ms")
+    gate = 0.99
+    line_extra = {}
+    if args.learned_only:
+        ri, gr, gi, cm = args.learned_schedule or (1, 1, 1, 0.0)
+        learned_s = init_net.LearnedSchedule(refine_iters=int(ri), geometric_rounds=int(gr), geometric_iters=int(gi), conf_min=cm)
+    else:
+        # the plain method's read-out on every view: the accuracy reference of every arm
+        p_outs, _, p_raw, _ = s.densify(views, plain_s)
+        rs_outs, _, rs_raw, _ = s.densify(views, dataclasses.replace(plain_s, first_run_id=101))
+        pn = [o.normal for o in p_outs]
+        acc_rs, nrm_rs, npx = agreement(p_outs, p_raw, rs_raw, pn, [o.normal for o in rs_outs])
+        del rs_outs, rs_raw
+        log(f"plain on {npx} valid pixels; plain with other random streams agrees on {acc_rs:.4f} (normals {nrm_rs:.4f})")
 
-            calib = views[:12]
+        def run_learned(c):
+            outs, _, raw, _, _, _ = init_net.densify_learned(s, net, views, init_net.LearnedSchedule(*c[:3], conf_min=c[3]))
+            return agreement(p_outs, p_raw, raw)[0], (outs, raw)
+
+        def run_random(c):
+            _, _, raw, _ = s.densify(views, abi.Schedule(photometric_iters=c[0], geometric_rounds=c[1], geometric_iters=c[2]))
+            return agreement(p_outs, p_raw, raw)[0], raw
+
+        if args.learned_schedule:
+            ri, gr, gi, cm = args.learned_schedule
+            lc = (int(ri), int(gr), int(gi), cm)
+            acc_l, (l_outs, l_raw) = run_learned(lc)
+            rc = (int(ri) + 1, int(gr), int(gi))
+            acc_r, r_raw = run_random(rc)
+            l_rows = r_rows = l_tried = r_tried = []
+        else:
+            # schedule autotuner on the calibration views, then confirmed on every view; the
+            # random-init control arm goes through the same two steps
             t = time.perf_counter()
-            learned_s, tune_rows = autotune.tune_learned(s, net, calib, plain_s)
-            learned_s = dataclasses.replace(learned_s, post=post)
-            log(f"schedule autotuner on {len(calib)} views ({time.perf_counter() - t:.1f} s): "
-                + ", ".join(f"{c}: {a:.4f} in {ms:.0f} ms" for c, a, ms in tune_rows)
-                + f" -> refine {learned_s.refine_iters}, geometric {learned_s.geometric_rounds}x{learned_s.geometric_iters}, "
-                  f"confidence {learned_s.conf_min}")
-        # the timed jobs end at the read-out threshold, as the metric does; the downloaded job that
-        # feeds the fusion adds the post-processing (densify.rs:455-462)
-        timed_p = dataclasses.replace(plain_s, post=None)
-        timed_l = dataclasses.replace(learned_s, post=None)
-        for name, run in [("plain", lambda dl: s.densify(views, plain_s if dl else timed_p, download=dl)),
-                          ("learned", lambda dl: init_net.densify_learned(s, net, views, learned_s if dl else timed_l, download=dl))]:
-            for i in range(args.warmup):
-                run(False)
-            device_sync()
-            t = time.perf_counter()
-            tms = []
-            for i in range(args.steps):
-                r = run(False)
-                tms.append(r[3])
-            device_sync()
-            ms = (time.perf_counter() - t) / args.steps * 1e3
-            log(f"{name}: {ms:.1f} ms per job, {job_mpx / (ms / 1e3):.2f} Mpx/s")
-            outs = run(True)
-            res[name] = (ms, outs, tms)
-        # two plain runs that differ only in their random streams: the agreement a method can have
-        reseeded = s.densify(views, abi.Schedule(photometric_iters=args.photometric_iters, geometric_rounds=args.geometric_rounds,
-                                                 geometric_iters=args.geometric_iters, first_run_id=101, post=post))
-    # the plain method's read-out mask: raw depth > 0 and count >= max(1, min(min_num_consistent, nsrc))
-    # (densify.rs:446-454), before any post-processing
-    _, fl = abi.fast_profile()
-    valid = [(res["plain"][1][2][v] > 0) & (res["plain"][1][1][v] >= max(1, min(fl.min_num_consistent, len(src[v])))) for v in views]
-    # accuracy of the learned maps against the plain ones (raw depth and final normal on that mask)
-    ok_d = ok_n = tot = 0
-    for v in views:
-        dp, dl = res["plain"][1][2][v], res["learned"][1][2][v]
-        npl, nle = res["plain"][1][0][v].normal, res["learned"][1][0][v].normal
-        m = valid[v]
-        tot += int(m.sum())
-        rel = np.abs(dl[m] - dp[m]) / dp[m]
-        ok_d += int((rel <= 0.01).sum())
-        cosang = np.clip((npl[m] * nle[m]).sum(-1), -1, 1)
-        ok_n += int((np.degrees(np.arccos(cosang)) <= 2.0).sum())
-    frac_d, frac_n = ok_d / max(tot, 1), ok_n / max(tot, 1)
-    log(f"learned vs plain: depth within 1 % on {frac_d:.4f}, normal within 2 deg on {frac_n:.4f} of {tot} px")
-    rd = rn = 0
-    for v in views:
-        m = valid[v]
-        dp, dr = res["plain"][1][2][v], reseeded[2][v]
-        rd += int((np.abs(dr[m] - dp[m]) <= 0.01 * dp[m]).sum())
-        cosang = np.clip((res["plain"][1][0][v].normal[m] * reseeded[0][v].normal[m]).sum(-1), -1, 1)
-        rn += int((np.degrees(np.arccos(cosang)) <= 2.0).sum())
-    base_d, base_n = rd / max(tot, 1), rn / max(tot, 1)
-    log(f"plain vs plain with other random streams: depth within 1 % on {base_d:.4f}, normal within 2 deg on {base_n:.4f}")
-    # fused clouds (min views 3) and their comparison
-    clouds = {}
-    for name in ("plain", "learned"):
-        outs = res[name][1][0]
-        keep = []
-        fv = fusion.fusion_views(sc, [o.depth for o in outs], [o.normal for o in outs], [o.cost for o in outs], src, src, keep)
-        t = time.perf_counter()
-        clouds[name] = fusion.fuse(be, fv, fusion.FusionParams(), int(sum((o.depth > 0).sum() for o in outs)) + 1)
-        log(f"{name} fusion: {len(clouds[name].positions)} points in {time.perf_counter() - t:.1f} s")
-    med, comp = fusion.compare(clouds["learned"], clouds["plain"], 0.05)
-    log(f"clouds: median learned->plain distance {med * 100:.2f} cm, completeness at 5 cm {comp:.4f}")
-    ms_l, ms_p = res["learned"][0], res["plain"][0]
+            ref = autotune.plain_reference(s, calib, plain_s)
+            _, l_rows = autotune.tune_learned(s, net, calib, plain_s, gate=gate, reference=ref)
+            _, r_rows = autotune.tune_random(s, calib, plain_s, gate=gate, reference=ref)
+            log(f"schedule autotuner on {len(calib)} views ({time.perf_counter() - t:.1f} s): learned "
+                + ", ".join(f"{c}: {a:.4f} {ms:.0f} ms" for c, a, ms in l_rows) + "; random "
+                + ", ".join(f"{c}: {a:.4f} {ms:.0f} ms" for c, a, ms in r_rows))
+            lc, acc_l, (l_outs, l_raw), l_tried = confirm(l_rows, gate, run_learned, "learned")
+            rc, acc_r, r_raw, r_tried = confirm(r_rows, gate, run_random, "random init")
+        learned_s = init_net.LearnedSchedule(refine_iters=lc[0], geometric_rounds=lc[1], geometric_iters=lc[2], conf_min=lc[3])
+        random_s = abi.Schedule(photometric_iters=rc[0], geometric_rounds=rc[1], geometric_iters=rc[2])
+        # the random-init twin of the learned schedule: the same iterations from the reference's start
+        twin_s = abi.Schedule(photometric_iters=learned_s.refine_iters, geometric_rounds=learned_s.geometric_rounds,
+                              geometric_iters=learned_s.geometric_iters)
+        acc_t, _ = run_random((twin_s.photometric_iters, twin_s.geometric_rounds, twin_s.geometric_iters))
+        nrm_l = agreement(p_outs, p_raw, l_raw, pn, [o.normal for o in l_outs])[1]
+        log(f"chosen: learned {lc} ({acc_l:.4f} on every view), random init {rc} ({acc_r:.4f}); random init on the learned "
+            f"schedule {acc_t:.4f}")
+        line_extra["autotune"] = {
+            "calibration_views": len(calib) if l_rows else 0,
+            "learned": [[list(c), round(a, 5), round(ms, 1)] for c, a, ms in l_rows],
+            "random": [[list(c), round(a, 5), round(ms, 1)] for c, a, ms in r_rows],
+            "confirmed_on_every_view": {"learned": [[list(c), round(a, 5)] for c, a in l_tried],
+                                        "random": [[list(c), round(a, 5)] for c, a in r_tried]}}
+
+    # ---- timed arms (device-resident: images uploaded at begin, states stay in HBM)
+    ms_l, tms_l = timed("learned", lambda: init_net.densify_learned(s, net, views, learned_s, download=False)[3],
+                        args.steps, args.warmup)
+    # roofline of the dominant kernel (the half step) from the learned arm's own launches
+    launches = [(0, False)] * (2 * learned_s.refine_iters) + [(0, True)] * (2 * learned_s.geometric_rounds * learned_s.geometric_iters)
+    avg_bytes, flop = half_step_work(inp, src, launches)
+    half_ms = sum(t.half_kernel_ms for t in tms_l) / max(sum(t.half_launches for t in tms_l), 1)
+    sched_key = f"zone{nviews}_{learned_s.refine_iters}_{learned_s.geometric_rounds}x{learned_s.geometric_iters}"
+    roof, roof_valu = roofline_objects("k_half<5,2>", avg_bytes, flop, half_ms, K_HALF_TRAFFIC.get(sched_key))
+    last = tms_l[-1]
     line = {
-        "metric": "depth-map Mpx/s at <=1% rel. depth error vs plain PatchStereo; HBM GB/s",
+        "metric": METRIC,
         "value": round(job_mpx / (ms_l / 1e3), 3),
         "unit": "Mpx/s",
         "n_gpus": 1,
@@ -258,76 +412,102 @@ def zone_main(args):
         "scaling": "weak",
         "vs_baseline": None,
         "dtype": "f32",
-        "data": "synthetic (seeded fBm terrain, rendered on the host)",
+        "data": "synthetic (seeded fBm terrain rendered on the device, sigma 0.01 noise, u8)",
         "config": {
-            "workload": f"zone: {nviews} views 1920x1080, 4-8 sources, learned initialiser (bf16 U-Net at 480x270) + "
-                        f"{learned_s.refine_iters} refinement iterations + {learned_s.geometric_rounds}x{learned_s.geometric_iters} geometric, "
-                        f"against plain PatchStereo ({args.photometric_iters} + {args.geometric_rounds}x{args.geometric_iters})",
+            "workload": f"zone: {nviews} views 1920x1080 ({stations} stations x 3 cameras), 4-8 sources each, learned initialiser "
+                        f"(bf16 U-Net at 480x270, 32-depth sweep) + {learned_s.refine_iters} refinement + "
+                        f"{learned_s.geometric_rounds}x{learned_s.geometric_iters} geometric iterations + read-out, "
+                        f"one launch per half step over all views; accuracy gate against plain PatchStereo "
+                        f"({args.photometric_iters} + {args.geometric_rounds}x{args.geometric_iters})",
             "views": nviews,
             "megapixels": round(job_mpx, 3),
+            "sources": [len(x) for x in src],
+            "tile": {"rows": shape[0], "lanes_per_pixel": shape[1], "autotuned": bool(args.tile_autotune)},
+            "learned_schedule": {"refine_iters": learned_s.refine_iters, "geometric_rounds": learned_s.geometric_rounds,
+                                 "geometric_iters": learned_s.geometric_iters, "conf_min": learned_s.conf_min},
         },
-        "plain_mpx_s": round(job_mpx / (ms_p / 1e3), 3),
-        "speedup_vs_plain": round(ms_p / ms_l, 3),
-        "accuracy_vs_plain": {"depth_within_1pct": round(frac_d, 5), "normal_within_2deg": round(frac_n, 5),
-                              "gate_depth_99pct": frac_d >= 0.99,
-                              "plain_reseeded_depth_within_1pct": round(base_d, 5),
-                              "plain_reseeded_normal_within_2deg": round(base_n, 5)},
-        "learned_schedule": {"refine_iters": learned_s.refine_iters, "geometric_rounds": learned_s.geometric_rounds,
-                             "geometric_iters": learned_s.geometric_iters, "conf_min": learned_s.conf_min,
-                             "autotuned_on_views": 12 if tune_rows else 0,
-                             "candidates": [[list(c), round(a, 5), round(ms, 1)] for c, a, ms in tune_rows] if tune_rows else None},
-        "stage_ms_learned": {"init": round(res["learned"][2][-1].init_ms, 2), "photometric": round(res["learned"][2][-1].photometric_ms, 2),
-                             "geometric": round(res["learned"][2][-1].geometric_ms, 2), "filter": round(res["learned"][2][-1].filter_ms, 2)},
-        "post_processing": {"speckles": bool(post), "fill_holes": bool(post), "timed": False,
-                            "plain_ms": round(res["plain"][1][3].post_ms, 2), "learned_ms": round(res["learned"][1][3].post_ms, 2),
-                            "plain_removed": int(res["plain"][1][3].post_removed), "plain_filled": int(res["plain"][1][3].post_filled),
-                            "learned_removed": int(res["learned"][1][3].post_removed),
-                            "learned_filled": int(res["learned"][1][3].post_filled)},
-        "clouds": {"plain_points": len(clouds["plain"].positions), "learned_points": len(clouds["learned"].positions),
-                   "median_distance_m": round(med, 5), "completeness_5cm": round(comp, 5)},
+        "roofline": roof,
+        "roofline_valu": roof_valu,
+        "stage_ms_learned": {"init": round(last.init_ms, 2), "photometric": round(last.photometric_ms, 2),
+                             "geometric": round(last.geometric_ms, 2), "filter": round(last.filter_ms, 2)},
     }
+    line.update(line_extra)
+    if args.learned_only:
+        s.close()
+        print(json.dumps(line), flush=True)
+        return
+    ms_p, _ = timed("plain", lambda: s.densify(views, plain_s, download=False)[3], args.plain_steps, 1)
+    ms_t, _ = timed("random init, learned schedule", lambda: s.densify(views, twin_s, download=False)[3], args.plain_steps, 1)
+    ms_r, _ = timed("random init, tuned schedule", lambda: s.densify(views, random_s, download=False)[3], args.plain_steps, 1)
+
+    # ---- fused clouds (min views 3) of the learned and plain maps, after the fast profile's
+    # read-out post-processing (speckle removal and hole filling, densify.rs:455-462), and their
+    # comparison; untimed
+    if args.fusion:
+        post = abi.PostParams.fast() if args.post else None
+        if post is not None:
+            del p_outs, l_outs
+            p_outs, _, _, p_tm = s.densify(views, dataclasses.replace(plain_s, post=post))
+            l_outs, _, _, l_tm, _, _ = init_net.densify_learned(s, net, views, dataclasses.replace(learned_s, post=post))
+            line["post_processing"] = {"speckles": True, "fill_holes": True, "timed": False,
+                                       "plain_ms": round(p_tm.post_ms, 2), "learned_ms": round(l_tm.post_ms, 2),
+                                       "plain_removed": int(p_tm.post_removed), "plain_filled": int(p_tm.post_filled),
+                                       "learned_removed": int(l_tm.post_removed), "learned_filled": int(l_tm.post_filled)}
+        clouds = {}
+        for name, outs in (("plain", p_outs), ("learned", l_outs)):
+            keep = []
+            fv = fusion.fusion_views(sc, [o.depth for o in outs], [o.normal for o in outs], [o.cost for o in outs], src, src, keep)
+            t = time.perf_counter()
+            clouds[name] = fusion.fuse(be, fv, fusion.FusionParams(), int(sum((o.depth > 0).sum() for o in outs)) + 1)
+            log(f"{name} fusion: {len(clouds[name].positions)} points in {time.perf_counter() - t:.1f} s")
+        del p_outs, l_outs
+        t = time.perf_counter()
+        med, comp = fusion.compare(clouds["learned"], clouds["plain"], 0.05)
+        log(f"clouds: median learned->plain distance {med * 100:.2f} cm, completeness at 5 cm {comp:.4f} "
+            f"({time.perf_counter() - t:.1f} s)")
+        line["clouds"] = {"plain_points": len(clouds["plain"].positions), "learned_points": len(clouds["learned"].positions),
+                          "median_distance_m": round(med, 5), "completeness_5cm": round(comp, 5), "min_views": 3,
+                          "post_processing": post is not None}
+        del clouds
+    s.close()
+
+    # ---- one learned job from c3d_pm_begin to the downloaded read-out (the host-boundary rate)
+    device_sync()
+    t = time.perf_counter()
+    with be.begin(inp) as s2:  # the backend's default tile shape (4, 1): no tuning work inside the timed region
+        init_net.densify_learned(s2, net, views, learned_s, download=True)
+    b2d_s = time.perf_counter() - t
+    log(f"learned begin -> downloaded read-out: {b2d_s * 1e3:.1f} ms ({job_mpx / b2d_s:.2f} Mpx/s)")
+    line.update({
+        "accuracy_vs_plain": {"depth_within_1pct": round(acc_l, 5), "gate_depth_99pct": acc_l >= gate,
+                              "normal_within_2deg": round(nrm_l, 5), "valid_plain_pixels": npx,
+                              "plain_reseeded_depth_within_1pct": round(acc_rs, 5),
+                              "plain_reseeded_normal_within_2deg": round(nrm_rs, 5)},
+        "plain": {"mpx_s": round(job_mpx / (ms_p / 1e3), 3), "ms_per_job": round(ms_p, 2), "steps": args.plain_steps,
+                  "schedule": [args.photometric_iters, args.geometric_rounds, args.geometric_iters]},
+        "speedup_vs_plain": round(ms_p / ms_l, 3),
+        "random_init_same_schedule": {"mpx_s": round(job_mpx / (ms_t / 1e3), 3), "ms_per_job": round(ms_t, 2),
+                                      "depth_within_1pct": round(acc_t, 5), "gate_depth_99pct": acc_t >= gate,
+                                      "schedule": [twin_s.photometric_iters, twin_s.geometric_rounds, twin_s.geometric_iters]},
+        "random_init_tuned": {"mpx_s": round(job_mpx / (ms_r / 1e3), 3), "ms_per_job": round(ms_r, 2),
+                              "depth_within_1pct": round(acc_r, 5), "gate_depth_99pct": acc_r >= gate,
+                              "schedule": [random_s.photometric_iters, random_s.geometric_rounds, random_s.geometric_iters]},
+        "speedup_vs_random_tuned": round(ms_r / ms_l, 3),
+        "begin_to_download": {"mpx_s": round(job_mpx / b2d_s, 3), "ms": round(b2d_s * 1e3, 1)},
+    })
+
+    net.close()
+    if not args.no_cpu:
+        line["cpu_baseline"] = cpu_baseline_object(inp, src, views, args.cpu_rows, args.photometric_iters, args.geometric_rounds,
+                                                   args.geometric_iters, job_mpx)
+    be.close()
     print(json.dumps(line), flush=True)
 
 
-def main():
-    ap = argparse.ArgumentParser()
-    ap.add_argument("--gpus", type=int, default=1)
-    ap.add_argument("--steps", type=int, default=3)
-    ap.add_argument("--warmup", type=int, default=1)
-    ap.add_argument("--workload", choices=["zone7", "batched48", "zone108", "fast960"], default="zone7",
-                    help="zone7: 7 views 1920x1080, 6 sources each (default); batched48: 48 views of uneven "
-                         "resolution and source count in one batch, schedule captured in a hipGraph")
-    ap.add_argument("--window", type=int, nargs=2, default=[5, 2], metavar=("RADIUS", "STEP"),
-                    help="default workload: NCC window radius and step (5 2 = fast profile, 5 1 = quality profile's window)")
-    ap.add_argument("--width", type=int, default=1920)
-    ap.add_argument("--height", type=int, default=1080)
-    ap.add_argument("--views", type=int, default=None)
-    ap.add_argument("--photometric-iters", type=int, default=7)
-    ap.add_argument("--geometric-rounds", type=int, default=1)
-    ap.add_argument("--geometric-iters", type=int, default=2)
-    ap.add_argument("--graph", type=int, default=None, help="1 = replay the schedule as one hipGraph (default: batched48 only)")
-    ap.add_argument("--refine-iters", type=int, default=2, help="zone108: refinement iterations after the learned init")
-    ap.add_argument("--autotune", type=int, default=1, help="zone108: pick the learned schedule with the autotuner (1) or not (0)")
-    ap.add_argument("--tile-autotune", type=int, default=1,
-                    help="zone7/batched48: pick the half step's workgroup shape with c3d_pm_autotune before timing (1) or keep (4, 1) (0)")
-    ap.add_argument("--post", type=int, default=1, help="zone108: the fast profile's speckle removal and hole filling (1) or none (0)")
-    ap.add_argument("--cpu-rows", type=int, default=320)
-    ap.add_argument("--no-cpu", action="store_true")
-    ap.add_argument("--calibrate-copy", type=int, default=0, metavar="MIB",
-                    help="also run a device-to-device copy of this many MiB (profiling calibration)")
-    args = ap.parse_args()
-    if args.gpus != 1:
-        print(f"bench.py measures one GPU only (every box here has one); --gpus {args.gpus} is not supported", file=sys.stderr)
-        sys.exit(2)
-    if not pm.is_available():
-        print("no gfx950 device: the engine has no CPU path", file=sys.stderr)
-        sys.exit(1)
+# ---------------------------------------------------------------- plain workloads
 
-    if args.calibrate_copy:
-        copy_calibration(args.calibrate_copy)
-    if args.workload == "zone108":
-        zone_main(args)
-        return
+
+def plain_main(args):
     batched = args.workload in ("batched48", "fast960")
     fast960 = args.workload == "fast960"
     use_graph = args.graph if args.graph is not None else int(batched)
@@ -347,7 +527,7 @@ def main():
     sizes = [inp.level_shape(v, top) for v in range(nviews)]
     npix_v = [h * w for (h, w) in sizes]
     job_mpx = sum(npix_v) / 1e6
-    log(f"scene {nviews} views, {job_mpx:.1f} Mpx rendered in {time.perf_counter() - t0:.1f} s")
+    log(f"scene {nviews} views, {job_mpx:.1f} Mpx rendered on the device in {time.perf_counter() - t0:.1f} s")
     views = list(range(nviews))
     if fast960:
         # LevelSchedule of the fast profile (params.rs:496-503): 6 / 3 photometric iterations,
@@ -395,31 +575,19 @@ def main():
         _, _, _, tm = sess.densify(views, sched, download=False)
         tms = [tm]
         log(f"eager run for kernel times: {tm.total_ms:.1f} ms")
-
-    # dominant kernel: the red-black half step. Algorithmic work per pixel-iteration (one pixel
-    # updated once by a red + black pair): S = 15 nsrc WN^2 bilinear samples (upper bound: 9
-    # propagated + 6 refinement hypotheses over every source), 28 FLOP each; compulsory HBM bytes
-    # 58 + 2 (1 + nsrc), plus 8 nsrc for geometric iterations. One launch updates npix/2 pixels
-    # of every view.
-    wn = (2 * inp.pm.window_radius) // inp.pm.window_step + 1
-    # every half-step launch of the schedule: (pixels of each view at its scale, geometric?)
     launches = []
     for lv in range(inp.num_levels):
-        pix_l = [h * w for (h, w) in (inp.level_shape(v, lv) for v in range(nviews))]
-        n_ph = 2 * (sched.photometric_iters if lv == 0 else sched.photometric_iters_upper)
-        launches += [(pix_l, False)] * n_ph + [(pix_l, True)] * (2 * sched.geometric_rounds * sched.geometric_iters)
-    avg_bytes = sum(sum(p / 2 * (58 + 2 * (1 + len(s)) + (8 * len(s) if geo else 0)) for p, s in zip(pix_l, src))
-                    for pix_l, geo in launches) / len(launches)
-    flop = sum(sum(FLOP_PER_SAMPLE * 15 * len(s) * wn * wn * p / 2 for p, s in zip(pix_l, src)) for pix_l, _ in launches) / len(launches)
+        n_ph = 2 * (sched.photometric_iters if lv == top else sched.photometric_iters_upper)
+        launches += [(lv, False)] * n_ph + [(lv, True)] * (2 * sched.geometric_rounds * sched.geometric_iters)
+    avg_bytes, flop = half_step_work(inp, src, launches)
     half_ms = sum(t.half_kernel_ms for t in tms) / sum(t.half_launches for t in tms)
-    achieved_gbs = avg_bytes / (half_ms / 1e3) / 1e9
     default_zone7 = (not batched and nviews == 7 and (args.width, args.height) == (1920, 1080) and args.photometric_iters == 7
                      and args.geometric_rounds == 1 and args.geometric_iters == 2 and tuple(args.window) == (5, 2))
-    achieved_tflops = flop / (half_ms / 1e3) / 1e12
     # the windows built with register-held weights; every other one runs the any-radius build
     reg_windows = {(5, 2), (4, 2), (3, 2), (2, 2), (3, 1), (2, 1), (1, 1)}
     kernel_name = (f"k_half<{inp.pm.window_radius},{inp.pm.window_step}>"
                    if (inp.pm.window_radius, inp.pm.window_step) in reg_windows else f"k_half<any radius,{inp.pm.window_step}>")
+    roof, roof_valu = roofline_objects(kernel_name, avg_bytes, flop, half_ms, K_HALF_TRAFFIC["zone7"] if default_zone7 else None)
     nsrc_desc = f"{len(src[0])} src each" if len({len(s) for s in src}) == 1 else \
         f"{min(len(s) for s in src)}-{max(len(s) for s in src)} src"
     size_desc = f"{sizes[0][1]}x{sizes[0][0]}" if len(set(sizes)) == 1 else \
@@ -431,7 +599,7 @@ def main():
         iters_desc = (f"{args.photometric_iters} photometric + {args.geometric_rounds}x{args.geometric_iters} "
                       f"geometric iterations")
     line = {
-        "metric": "depth-map Mpx/s at <=1% rel. depth error vs plain PatchStereo; HBM GB/s",
+        "metric": METRIC,
         "value": round(value, 3),
         "unit": "Mpx/s",
         "n_gpus": 1,
@@ -442,7 +610,7 @@ def main():
         "scaling": "weak",
         "vs_baseline": None,
         "dtype": "f32",
-        "data": "synthetic (seeded fBm terrain, rendered on the host)",
+        "data": "synthetic (seeded fBm terrain rendered on the device, sigma 0.01 noise, u8)",
         "config": {
             "workload": f"plain PatchStereo, {nviews} views {size_desc} u8, 1 ref + {nsrc_desc}, "
                         f"window {inp.pm.window_radius} step {inp.pm.window_step}, {iters_desc}, read-out filter, depth 15-120 m"
@@ -457,26 +625,8 @@ def main():
             "graph": use_graph,
             "tile": {"rows": shape[0], "lanes_per_pixel": shape[1], "autotuned": bool(args.tile_autotune)},
         },
-        "roofline": {
-            "bound": "hbm",
-            "kernel": kernel_name,
-            "achieved": round(achieved_gbs, 2),
-            "peak": HBM_PEAK_GBS,
-            "unit": "GB/s",
-            "frac": round(achieved_gbs / HBM_PEAK_GBS, 5),
-            "traffic": K_HALF_TRAFFIC_ZONE7 if default_zone7 else None,
-            "avg_launch_ms": round(half_ms, 3),
-            "algorithmic_bytes_per_launch": int(avg_bytes),
-        },
-        "roofline_valu": {
-            "bound": "valu-fp32",
-            "kernel": kernel_name,
-            "achieved": round(achieved_tflops, 2),
-            "peak": FP32_PEAK_TFLOPS,
-            "unit": "TFLOP/s",
-            "frac": round(achieved_tflops / FP32_PEAK_TFLOPS, 4),
-            "algorithmic_flop_per_launch": int(flop),
-        },
+        "roofline": roof,
+        "roofline_valu": roof_valu,
         "stage_ms": {
             "photometric": round(sum(t.photometric_ms for t in tms) / len(tms), 2),
             "geometric": round(sum(t.geometric_ms for t in tms) / len(tms), 2),
@@ -484,23 +634,60 @@ def main():
         },
     }
     sess.close()
-    if not args.no_cpu and not fast960:
-        threads = min(os.cpu_count() or 1, 16)
-        view = max(range(nviews), key=lambda v: npix_v[v])
-        rate, dt, sample = cpu_baseline(inp, args.cpu_rows, threads, args.photometric_iters, view)
-        job_px_iters = sum(npix_v) * (args.photometric_iters + 1 + args.geometric_rounds * (args.geometric_iters + 1) + 1)
-        cpu_mpxs = job_mpx / (job_px_iters / rate)
-        line["cpu_baseline"] = {
-            "value": round(cpu_mpxs, 5),
-            "unit": "Mpx/s",
-            "cores": threads,
-            "kind": "port",
-            "sample": f"{sample}: {dt:.1f} s, {rate:.0f} px-iter/s, scaled to the job's "
-                      f"{job_px_iters / 1e6:.1f} M px-iters (the reference has no CPU PatchStereo; this is the C oracle, "
-                      f"{platform.processor() or platform.machine()})",
-        }
-        log(f"cpu baseline: {rate:.0f} px-iter/s on {threads} threads ({dt:.1f} s)")
+    if not args.no_cpu and not fast960 and inp.num_levels == 1:
+        line["cpu_baseline"] = cpu_baseline_object(inp, src, views, args.cpu_rows, args.photometric_iters, args.geometric_rounds,
+                                                   args.geometric_iters, job_mpx)
     print(json.dumps(line), flush=True)
+
+
+def parse_args(argv=None):
+    ap = argparse.ArgumentParser()
+    ap.add_argument("--gpus", type=int, default=1)
+    ap.add_argument("--steps", type=int, default=5)
+    ap.add_argument("--warmup", type=int, default=1)
+    ap.add_argument("--workload", choices=["zone108", "zone7", "batched48", "fast960"], default="zone108",
+                    help="zone108: 108 views 1920x1080, learned init + batched PatchStereo + geometric round (default); "
+                         "zone7: 7 views 1920x1080, plain, 6 sources each; batched48: 48 views of uneven resolution and "
+                         "source count in one batch, hipGraph; fast960: fast profile over 3 scales, 240 views 960x540")
+    ap.add_argument("--window", type=int, nargs=2, default=[5, 2], metavar=("RADIUS", "STEP"),
+                    help="zone7: NCC window radius and step (5 2 = fast profile, 5 1 = quality profile's window)")
+    ap.add_argument("--width", type=int, default=1920)
+    ap.add_argument("--height", type=int, default=1080)
+    ap.add_argument("--views", type=int, default=None)
+    ap.add_argument("--photometric-iters", type=int, default=7)
+    ap.add_argument("--geometric-rounds", type=int, default=1)
+    ap.add_argument("--geometric-iters", type=int, default=2)
+    ap.add_argument("--graph", type=int, default=None, help="1 = replay the schedule as one hipGraph (default: batched48, fast960)")
+    ap.add_argument("--tile-autotune", type=int, default=1,
+                    help="zone7/batched48: pick the half step's workgroup shape with c3d_pm_autotune before timing (1) or keep (4, 1) (0)")
+    ap.add_argument("--plain-steps", type=int, default=2, help="zone108: timed jobs of the plain and random-init arms")
+    ap.add_argument("--learned-only", action="store_true", help="zone108: time the learned arm only (profiling runs)")
+    ap.add_argument("--learned-schedule", type=float, nargs=4, default=None, metavar=("REFINE", "GROUNDS", "GITERS", "CONF"),
+                    help="zone108: fixed learned schedule instead of the autotuner's choice (profiling runs)")
+    ap.add_argument("--fusion", type=int, default=1, help="zone108: fuse and compare the clouds (1) or not (0)")
+    ap.add_argument("--post", type=int, default=1,
+                    help="zone108: the fast profile's speckle removal and hole filling before fusion (1) or none (0)")
+    ap.add_argument("--cpu-rows", type=int, default=400)
+    ap.add_argument("--no-cpu", action="store_true")
+    ap.add_argument("--calibrate-copy", type=int, default=0, metavar="MIB",
+                    help="also run a device-to-device copy of this many MiB (profiling calibration)")
+    return ap.parse_args(argv)
+
+
+def main():
+    args = parse_args()
+    if args.gpus != 1:
+        print(f"bench.py measures one GPU only (every box here has one); --gpus {args.gpus} is not supported", file=sys.stderr)
+        sys.exit(2)
+    if not pm.is_available():
+        print("no gfx950 device: the engine has no CPU path", file=sys.stderr)
+        sys.exit(1)
+    if args.calibrate_copy:
+        copy_calibration(args.calibrate_copy)
+    if args.workload == "zone108":
+        zone_main(args)
+    else:
+        plain_main(args)
 
 
 if __name__ == "__main__":

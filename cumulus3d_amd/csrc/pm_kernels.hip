@@ -845,6 +845,17 @@ const KernelSet* kernel_set(int wr, int wstep, bool any_radius) {
     return &kAny[wstep - 1];
 }
 
+// The per-sample reciprocal of the window loops, applied to n given values (c3d_pm_sample_reciprocal).
+__global__ __launch_bounds__(256) void k_sample_reciprocal(const float* z, float* r, long long n) {
+    const long long i = (long long)blockIdx.x * 256 + threadIdx.x;
+    if (i < n) r[i] = rcp_sample(z[i]);
+}
+
+hipError_t launch_sample_reciprocal(const float* z, float* r, long long n, hipStream_t s) {
+    hipLaunchKernelGGL(k_sample_reciprocal, dim3((unsigned)((n + 255) / 256)), dim3(256), 0, s, z, r, n);
+    return hipGetLastError();
+}
+
 hipError_t launch_upsample(const KParams& P, hipStream_t s) {
     hipLaunchKernelGGL(k_upsample, dim3(P.ntiles), dim3(256), 0, s, P);
     return hipGetLastError();

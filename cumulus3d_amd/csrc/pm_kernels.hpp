@@ -28,6 +28,7 @@ bool half_shape_ok(int rows, int tpp);
 int half_tiles(const BatchItem* items, int n, int rows, int tpp);
 int pix_tiles(const BatchItem* items, int n);
 
+hipError_t launch_sample_reciprocal(const float* z, float* r, long long n, hipStream_t s);
 hipError_t launch_upsample(const KParams& P, hipStream_t s);
 hipError_t launch_median(const KParams& P, hipStream_t s);
 hipError_t launch_quad_build(const uint8_t* g, int w, int h, uint32_t* q, hipStream_t s);

@@ -1607,4 +1607,21 @@ int c3d_pm_pair_costs(c3d_pm_session* s, uint32_t level, uint32_t view, uint32_t
     return C3D_OK;
 }
 
+int c3d_pm_sample_reciprocal(c3d_pm_backend* be, const float* z, float* r, uint64_t n) {
+    if (!be || (n && (!z || !r))) return fail(C3D_INVALID_ARGUMENT, "null argument");
+    if (n == 0) return C3D_OK;
+    if (n > (1ULL << 32)) return fail(C3D_INVALID_ARGUMENT, "more than 2^32 values in one call");
+    int rc = set_device(be);
+    if (rc) return rc;
+    hipStream_t st = be->stream;
+    DevBuf<float> dz, dr;
+    HIP_TRY(dz.reserve((size_t)n));
+    HIP_TRY(dr.reserve((size_t)n));
+    HIP_TRY(hipMemcpyAsync(dz.p, z, (size_t)n * sizeof(float), hipMemcpyHostToDevice, st));
+    HIP_TRY(launch_sample_reciprocal(dz.p, dr.p, (long long)n, st));
+    HIP_TRY(hipMemcpyAsync(r, dr.p, (size_t)n * sizeof(float), hipMemcpyDeviceToHost, st));
+    HIP_TRY(hipStreamSynchronize(st));
+    return C3D_OK;
+}
+
 }  // extern "C"

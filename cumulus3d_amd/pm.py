@@ -28,7 +28,7 @@ EXPORTS = [
     "c3d_pm_create", "c3d_pm_name", "c3d_pm_destroy", "c3d_pm_is_available", "c3d_pm_begin", "c3d_pm_run",
     "c3d_pm_evaluate", "c3d_pm_filter", "c3d_pm_upsample", "c3d_pm_median", "c3d_pm_end", "c3d_pm_last_error",
     "c3d_pm_densify", "c3d_pm_pair_costs", "c3d_pm_num_levels", "c3d_pm_build_pyramid", "c3d_pm_autotune",
-    "c3d_remove_speckles", "c3d_fill_holes", "c3d_fill_holes_noalloc",
+    "c3d_remove_speckles", "c3d_fill_holes", "c3d_fill_holes_noalloc", "c3d_pm_sample_reciprocal",
 ]
 
 
@@ -91,9 +91,10 @@ def load() -> C.CDLL:
         L.c3d_fill_holes.argtypes = [C.c_void_p, fp, fp, fp, fp, C.POINTER(C.c_uint8), C.POINTER(C.c_double), C.c_uint32,
                                      C.c_uint32, C.POINTER(abi.c3d_post_params), C.POINTER(C.c_uint64)]
         L.c3d_fill_holes_noalloc.argtypes = L.c3d_fill_holes.argtypes
+        L.c3d_pm_sample_reciprocal.argtypes = [C.c_void_p, fp, fp, C.c_uint64]
         for n in ("c3d_pm_create", "c3d_pm_begin", "c3d_pm_run", "c3d_pm_evaluate", "c3d_pm_filter", "c3d_pm_upsample",
                   "c3d_pm_median", "c3d_pm_densify", "c3d_pm_pair_costs", "c3d_pm_is_available", "c3d_pm_build_pyramid",
-                  "c3d_remove_speckles", "c3d_fill_holes", "c3d_fill_holes_noalloc"):
+                  "c3d_remove_speckles", "c3d_fill_holes", "c3d_fill_holes_noalloc", "c3d_fill_holes_noalloc"):
             getattr(L, n).restype = C.c_int
         _libs["engine"] = L
         return L
@@ -189,6 +190,13 @@ class HipPatchMatch:
         _check(self.L, self.L.c3d_fill_holes(self.h, _fp(depth), _fp(normal), _fp(cost), _fp(raw_cost),
                                              gray.ctypes.data_as(C.POINTER(C.c_uint8)), kk, w, h, C.byref(p), C.byref(n)))
         return int(n.value)
+
+    def sample_reciprocal(self, z: np.ndarray) -> np.ndarray:
+        """The kernels' per-sample reciprocal 1/z on the device (c3d_pm_sample_reciprocal)."""
+        z = np.ascontiguousarray(z, np.float32)
+        r = np.zeros_like(z)
+        _check(self.L, self.L.c3d_pm_sample_reciprocal(self.h, _fp(z), _fp(r), z.size))
+        return r
 
     def close(self):
         if self.h:

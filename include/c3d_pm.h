@@ -394,6 +394,12 @@ int c3d_pm_build_pyramid(c3d_pm_backend* backend, const uint8_t* gray, uint32_t 
 int c3d_pm_pair_costs(c3d_pm_session* session, uint32_t level, uint32_t view, uint32_t src_slot,
                       uint32_t num_queries, const int32_t* xy, const float* planes, float* costs);
 
+/* The per-sample reciprocal of the projective divide inside every window loop (in place of the
+ * reference's `1.0f / zz`, patchmatch.cu:273), applied on the device of `backend` to n values (at
+ * most 2^32): r[i] = the kernels' 1/z[i]. It exists so that this arithmetic can be checked against
+ * the host restatement and IEEE division over whole binades. */
+int c3d_pm_sample_reciprocal(c3d_pm_backend* backend, const float* z, float* r, uint64_t n);
+
 #ifdef __cplusplus
 }
 #endif

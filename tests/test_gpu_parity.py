@@ -91,6 +91,23 @@ def test_pair_costs_match_fp64(c1):
                 assert ((ref == 2.0) != (got == 2.0)).sum() <= 2
 
 
+def test_sample_reciprocal_matches_the_oracle_over_whole_binades():
+    """The window loops' per-sample reciprocal on the device (c3d_pm_sample_reciprocal) equals the
+    oracle's restatement bit for bit on every float of [1, 2) and on 2^22 values over exponents
+    -40..40, and stays within 1 ulp of IEEE division (the reference's `1.0f / zz`)."""
+    import oracle
+
+    be = pm.HipPatchMatch()
+    z = np.arange(0x3F800000, 0x40000000, dtype=np.uint32).view(np.float32)
+    rng = np.random.default_rng(5)
+    z = np.concatenate([z, np.exp2(rng.uniform(-40, 40, 1 << 22)).astype(np.float32)])
+    got = be.sample_reciprocal(z)
+    assert np.array_equal(got.view(np.uint32), oracle.rcp_sample(z).view(np.uint32))
+    ulp = np.abs(got.view(np.int32).astype(np.int64) - (np.float32(1.0) / z).view(np.int32).astype(np.int64))
+    assert ulp.max() <= 1
+    be.close()
+
+
 def test_pair_costs_bit_exact(c1):
     sc, inp = c1
     be = pm.HipPatchMatch()
