@@ -370,69 +370,116 @@ __device__ __forceinline__ void stage_pix_tile(float* tile, const uint8_t* g, in
 }
 
 // ================================================================ run start / evaluation (patchmatch.cu:659-692)
-template <int WR, int WSTEP>
+//
+// A run starts with pm_eval: random initialisation (mode 1, by random_init), the top-K cost of
+// every pixel's plane and the best view reset. When the run then has at least one iteration, the
+// costs of the pixels of colour 0 are dead: the first half step (colour 0) recomputes and rewrites
+// every colour-0 pixel without reading its stored cost (its candidate step reads only the 72
+// neighbours of the opposite colour), and nothing else reads them in between. So such a run starts
+// with k_eval_init over every pixel (the initialisation and best-view reset, no window) and
+// k_eval<ONE = true> for the pixels of colour 1 only, with the half step's one-colour lane
+// mapping, which does half the window work. Runs of no iterations and c3d_pm_evaluate keep the
+// whole evaluation (ONE = false).
+__device__ __forceinline__ float4 eval_init_plane(const KParams& P, const ViewDesc& vd, const LevelDesc& R, int x, int y, int pix,
+                                                  long long base, int mode) {
+    float4 h = P.plane[base + pix];
+    // random_init 1: every pixel (patchmatch.cu:668-676); 2: pixels without a positive depth
+    if (mode == 1 && (P.random_init == 1 || (P.random_init == 2 && !(h.w > 0.f)))) {
+        unsigned long long st = rng_state(P.seed, vd.key, P.level, P.run_id, 0xFFFF, 0, (unsigned long long)pix);
+        float d = random_depth(st, vd.dmin, vd.dmax);
+        f3 n = random_normal(st, pixel_ray(R, x, y));
+        h = mk4(n, d);
+        P.plane[base + pix] = h;
+    }
+    return h;
+}
+
+// top-K mean of the plane's per-source costs (patchmatch.cu:326-345), 2 for a non-positive depth
+template <int WR, int WSTEP, class REF>
+__device__ __forceinline__ float eval_cost(const KParams& P, const ViewDesc& vd, const LevelDesc& R, const PairRec* prs,
+                                           const LevelPair* lps, const REF& ref, int wr, int x, int y, float4 h) {
+    if (!(h.w > 0.f)) return 2.f;
+    RefWinT<WR, WSTEP> rw;
+    ref_stats_any<WR, WSTEP>(P, ref, x, y, wr, rw);
+    float b0 = 1e30f, b1 = 1e30f, b2 = 1e30f;
+#pragma nounroll
+    for (int j = 0; j < vd.nsrc; ++j) {
+        const PairRec& pr = prs[j];
+        const LevelDesc S = P.lv[pr.src * P.nlevels + P.level];
+        float gq = 0.f;
+        float cc = pair_cost<WR, WSTEP>(P, rw, ref, R, S, pr, lps[j], x, y, h, &gq);
+        if (P.geometric) cc = cc + P.lambda * gq;
+        if (cc < b0) {
+            b2 = b1;
+            b1 = b0;
+            b0 = cc;
+        } else if (cc < b1) {
+            b2 = b1;
+            b1 = cc;
+        } else if (cc < b2) {
+            b2 = cc;
+        }
+    }
+    int k = P.topk < vd.nsrc ? P.topk : vd.nsrc;
+    float s = b0;
+    if (k >= 2) s += b1;
+    if (k >= 3) s += b2;
+    return k > 0 ? s / (float)k : 2.0f;
+}
+
+// ONE = false: every pixel of 64 x 4 tiles, initialisation (mode 1) and cost. ONE = true: the
+// pixels of colour P.color of 128 x 4 spans (one lane per pixel, as the half step), cost only
+// (k_eval_init has initialised them).
+template <int WR, int WSTEP, bool ONE>
 __global__ __launch_bounds__(256) void k_eval(KParams P, int mode) {
-    constexpr int TW = PIX_TW + 2 * WR;
+    constexpr int SPAN = ONE ? HALF_SPAN : PIX_TW;
+    constexpr int TW = SPAN + 2 * WR;
     __shared__ float tile_s[WR > 0 ? TW * (PIX_TH + 2 * WR) : 1];
     extern __shared__ float dyn_lds[];
     const int wr = WR > 0 ? WR : P.wr;
     float* tile = WR > 0 ? tile_s : dyn_lds;
     BatchItem it;
     int px0, py0;
-    if (!locate_tile(P, xcd_tile(blockIdx.x, P.ntiles), PIX_TW, PIX_TH, it, px0, py0)) return;
+    if (!locate_tile(P, xcd_tile(blockIdx.x, P.ntiles), SPAN, PIX_TH, it, px0, py0)) return;
     const LevelDesc R = P.lv[it.view * P.nlevels + P.level];
-    stage_pix_tile(tile, P.gray + R.goff, it.w, it.h, px0, py0, wr);
+    {
+        const uint8_t* g = P.gray + R.goff;
+        const int twd = SPAN + 2 * wr, thd = PIX_TH + 2 * wr;
+        const int tx0 = px0 - wr, ty0 = py0 - wr;
+        for (int i = threadIdx.x; i < twd * thd; i += 256) {
+            int ty = i / twd, tx = i - ty * twd;
+            int gx = tx0 + tx, gy = ty0 + ty;
+            tile[i] = (gx >= 0 && gy >= 0 && gx < it.w && gy < it.h) ? (float)g[(size_t)gy * it.w + gx] : 0.0f;
+        }
+    }
     __syncthreads();
-    const int x = px0 + (threadIdx.x & 63), y = py0 + (threadIdx.x >> 6);
+    const int lane = threadIdx.x & 63, y = py0 + (threadIdx.x >> 6);
+    const int x = ONE ? px0 + 2 * lane + ((y + P.color) & 1) : px0 + lane;
     if (x >= it.w || y >= it.h) return;
     const ViewDesc vd = P.views[it.view];
     const PairRec* prs = P.pairs + (size_t)it.view * MAXSRC;
     const LevelPair* lps = P.lpairs + ((size_t)it.view * P.nlevels + P.level) * MAXSRC;
     const int pix = y * it.w + x;
     const long long base = it.off;
-    f3 ray = pixel_ray(R, x, y);
-    float4 h = P.plane[base + pix];
-    // random_init 1: every pixel (patchmatch.cu:668-676); 2: pixels without a positive depth
-    if (mode == 1 && (P.random_init == 1 || (P.random_init == 2 && !(h.w > 0.f)))) {
-        unsigned long long st = rng_state(P.seed, vd.key, P.level, P.run_id, 0xFFFF, 0, (unsigned long long)pix);
-        float d = random_depth(st, vd.dmin, vd.dmax);
-        f3 n = random_normal(st, ray);
-        h = mk4(n, d);
-        P.plane[base + pix] = h;
-    }
-    float c = 2.f;
-    if (h.w > 0.f) {
-        const auto ref = tile_ref<WR, TW>(tile, px0 - wr, py0 - wr, PIX_TW + 2 * wr);
-        RefWinT<WR, WSTEP> rw;
-        ref_stats_any<WR, WSTEP>(P, ref, x, y, wr, rw);
-        // top-K mean (patchmatch.cu:326-345)
-        float b0 = 1e30f, b1 = 1e30f, b2 = 1e30f;
-#pragma nounroll
-        for (int j = 0; j < vd.nsrc; ++j) {
-            const PairRec& pr = prs[j];
-            const LevelDesc S = P.lv[pr.src * P.nlevels + P.level];
-            float gq = 0.f;
-            float cc = pair_cost<WR, WSTEP>(P, rw, ref, R, S, pr, lps[j], x, y, h, &gq);
-            if (P.geometric) cc = cc + P.lambda * gq;
-            if (cc < b0) {
-                b2 = b1;
-                b1 = b0;
-                b0 = cc;
-            } else if (cc < b1) {
-                b2 = b1;
-                b1 = cc;
-            } else if (cc < b2) {
-                b2 = cc;
-            }
-        }
-        int k = P.topk < vd.nsrc ? P.topk : vd.nsrc;
-        float s = b0;
-        if (k >= 2) s += b1;
-        if (k >= 3) s += b2;
-        c = k > 0 ? s / (float)k : 2.0f;
-    }
-    P.cost[base + pix] = c;
-    if (mode == 1) P.bview[base + pix] = 255;
+    const float4 h = ONE ? P.plane[base + pix] : eval_init_plane(P, vd, R, x, y, pix, base, mode);
+    const auto ref = tile_ref<WR, TW>(tile, px0 - wr, py0 - wr, SPAN + 2 * wr);
+    P.cost[base + pix] = eval_cost<WR, WSTEP>(P, vd, R, prs, lps, ref, wr, x, y, h);
+    if (!ONE && mode == 1) P.bview[base + pix] = 255;
+}
+
+// The window-free part of a run start for every pixel: random initialisation by random_init and
+// the best-view reset (patchmatch.cu:668-677). Stored costs are not touched.
+__global__ __launch_bounds__(256) void k_eval_init(KParams P) {
+    BatchItem it;
+    int px0, py0;
+    if (!locate_tile(P, blockIdx.x, PIX_TW, PIX_TH, it, px0, py0)) return;
+    const int x = px0 + (threadIdx.x & 63), y = py0 + (threadIdx.x >> 6);
+    if (x >= it.w || y >= it.h) return;
+    const LevelDesc R = P.lv[it.view * P.nlevels + P.level];
+    const ViewDesc vd = P.views[it.view];
+    const int pix = y * it.w + x;
+    eval_init_plane(P, vd, R, x, y, pix, it.off, 1);
+    P.bview[it.off + pix] = 255;
 }
 
 // ================================================================ final read-out (patchmatch.cu:694-737)
@@ -804,7 +851,13 @@ static hipError_t launch_half(const KParams& P0, int rows, int tpp, hipStream_t 
 }
 template <int WR, int WSTEP>
 static hipError_t launch_eval(const KParams& P, int mode, hipStream_t s) {
-    hipLaunchKernelGGL((k_eval<WR, WSTEP>), dim3(P.ntiles), dim3(256), WR == 0 ? pix_tile_bytes(P.wr) : 0, s, P, mode);
+    hipLaunchKernelGGL((k_eval<WR, WSTEP, false>), dim3(P.ntiles), dim3(256), WR == 0 ? pix_tile_bytes(P.wr) : 0, s, P, mode);
+    return hipGetLastError();
+}
+// P.ntiles must count 128 x 4 spans (half_tiles(..., 4, 1)); P.color selects the pixels
+template <int WR, int WSTEP>
+static hipError_t launch_eval_color(const KParams& P, hipStream_t s) {
+    hipLaunchKernelGGL((k_eval<WR, WSTEP, true>), dim3(P.ntiles), dim3(256), WR == 0 ? half_tile_bytes(P.wr, PIX_TH, 1) : 0, s, P, 0);
     return hipGetLastError();
 }
 template <int WR, int WSTEP>
@@ -825,8 +878,8 @@ static const void* half_symbol() {
 
 template <int WR, int WSTEP>
 static constexpr KernelSet make_set() {
-    return KernelSet{WR, WSTEP, &launch_half<WR, WSTEP>, &launch_eval<WR, WSTEP>, &launch_filter<WR, WSTEP>,
-                     &launch_pair_costs<WR, WSTEP>, &half_symbol<WR, WSTEP>};
+    return KernelSet{WR, WSTEP, &launch_half<WR, WSTEP>, &launch_eval<WR, WSTEP>, &launch_eval_color<WR, WSTEP>,
+                     &launch_filter<WR, WSTEP>, &launch_pair_costs<WR, WSTEP>, &half_symbol<WR, WSTEP>};
 }
 
 // Window variants compiled into the library (radius, step): the fast profile (5, 2) and the
@@ -853,6 +906,12 @@ __global__ __launch_bounds__(256) void k_sample_reciprocal(const float* z, float
 
 hipError_t launch_sample_reciprocal(const float* z, float* r, long long n, hipStream_t s) {
     hipLaunchKernelGGL(k_sample_reciprocal, dim3((unsigned)((n + 255) / 256)), dim3(256), 0, s, z, r, n);
+    return hipGetLastError();
+}
+
+// P.ntiles must count 64 x 4 tiles (pix_tiles)
+hipError_t launch_eval_init(const KParams& P, hipStream_t s) {
+    hipLaunchKernelGGL(k_eval_init, dim3(P.ntiles), dim3(256), 0, s, P);
     return hipGetLastError();
 }
 

@@ -10,6 +10,8 @@ struct KernelSet {
     int wr, wstep;
     hipError_t (*half)(const KParams&, int rows, int tpp, hipStream_t);
     hipError_t (*eval)(const KParams&, int mode, hipStream_t);
+    // cost of the pixels of colour P.color only (P.ntiles = half_tiles(items, 4, 1)), no initialisation
+    hipError_t (*eval_color)(const KParams&, hipStream_t);
     hipError_t (*filter)(const KParams&, hipStream_t);
     hipError_t (*pair_costs)(const KParams&, int view, int slot, int n, const int* xy, const float4* planes, float* out,
                              hipStream_t);
@@ -29,6 +31,8 @@ int half_tiles(const BatchItem* items, int n, int rows, int tpp);
 int pix_tiles(const BatchItem* items, int n);
 
 hipError_t launch_sample_reciprocal(const float* z, float* r, long long n, hipStream_t s);
+// random initialisation by P.random_init and best-view reset of every pixel (P.ntiles = pix_tiles)
+hipError_t launch_eval_init(const KParams& P, hipStream_t s);
 hipError_t launch_upsample(const KParams& P, hipStream_t s);
 hipError_t launch_median(const KParams& P, hipStream_t s);
 hipError_t launch_quad_build(const uint8_t* g, int w, int h, uint32_t* q, hipStream_t s);

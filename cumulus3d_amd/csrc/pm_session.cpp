@@ -392,7 +392,17 @@ int launch_run(c3d_pm_session* s, KParams k, const std::vector<BatchItem>& items
                uint32_t iterations, bool timed) {
     hipStream_t st = s->be->stream;
     k.ntiles = pix_tiles(items.data(), (int)items.size());
-    HIP_TRY(s->ks->eval(k, 1, st));
+    if (iterations == 0) {
+        HIP_TRY(s->ks->eval(k, 1, st));
+    } else {
+        // the colour-0 costs of the run start are overwritten unread by the first half step: only
+        // colour 1 is costed (k_eval_init initialises every pixel and resets the best view)
+        HIP_TRY(launch_eval_init(k, st));
+        KParams e = k;
+        e.color = 1;
+        e.ntiles = half_tiles(items.data(), (int)items.size(), 4, 1);
+        HIP_TRY(s->ks->eval_color(e, st));
+    }
     k.ntiles = half_tiles(items.data(), (int)items.size(), s->half_rows, s->half_tpp);
     for (uint32_t t = 0; t < iterations; ++t) {
         iteration_constants(pm, geometric, t, k);
