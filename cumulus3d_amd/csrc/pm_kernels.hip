@@ -909,6 +909,24 @@ hipError_t launch_sample_reciprocal(const float* z, float* r, long long n, hipSt
     return hipGetLastError();
 }
 
+// planes (nx, ny, nz, d) split into a depth array and a 3-float normal array (result downloads)
+__global__ __launch_bounds__(256) void k_split_planes(const float4* plane, float* depth, float* normal, long long n) {
+    const long long i = (long long)blockIdx.x * 256 + threadIdx.x;
+    if (i < n) {
+        const float4 q = plane[i];
+        depth[i] = q.w;
+        normal[3 * i] = q.x;
+        normal[3 * i + 1] = q.y;
+        normal[3 * i + 2] = q.z;
+    }
+}
+
+hipError_t launch_split_planes(const float4* plane, float* depth, float* normal, long long n, hipStream_t s) {
+    if (n <= 0) return hipSuccess;
+    hipLaunchKernelGGL(k_split_planes, dim3((unsigned)((n + 255) / 256)), dim3(256), 0, s, plane, depth, normal, n);
+    return hipGetLastError();
+}
+
 // P.ntiles must count 64 x 4 tiles (pix_tiles)
 hipError_t launch_eval_init(const KParams& P, hipStream_t s) {
     hipLaunchKernelGGL(k_eval_init, dim3(P.ntiles), dim3(256), 0, s, P);

@@ -182,6 +182,8 @@ struct c3d_pm_session {
     DevBuf<float4> low, up;
     DevBuf<float> cinit;
     DevBuf<uint8_t> bview, count, need;
+    // downloads: the final planes split into depth and 3-float normal arrays
+    DevBuf<float> dl_depth, dl_normal;
     // probes
     DevBuf<int> q_xy;
     DevBuf<float4> q_planes;
@@ -337,6 +339,13 @@ void unpack(const std::vector<float4>& h, const std::vector<BatchItem>& items, c
         }
     }
 }
+
+// The top-scale results of a densify call copied straight into the caller's arrays: the planes
+// are split into depth and 3-float normal arrays on the device, then each view's depth, normal,
+// cost, raw depth and count come back with one transfer each, in stream order, and one final
+// synchronisation (no host staging buffer, no host-side unpacking).
+int download_results(c3d_pm_session* s, const BatchItem* items, uint32_t n, long long npix, c3d_view_state* out,
+                     uint8_t* const* counts, float* const* raw_depth);
 
 // Snapshot upload, only when the id changes (patchmatch.rs:573-604).
 // The snapshot's maps are laid out with the sizes of snap->level and read with the sizes of the
@@ -720,6 +729,8 @@ void c3d_pm_end(c3d_pm_session* s) {
     s->post = nullptr;
     s->d_batch_lv.release();
     s->d_snap_lv.release();
+    s->dl_depth.release();
+    s->dl_normal.release();
     c3d_pm_backend* be = s->be;
     delete s;  // unlocks the backend's session mutex
     c3d::backend_release(be);
@@ -1221,31 +1232,8 @@ int c3d_pm_densify(c3d_pm_session* s, const uint32_t* views, uint32_t n, const c
     const long long npix = pl.npix[top];
     rc = run_post(s, sch->post, sch->post_k, items, n, timing);
     if (rc) return rc;
-    if (out) {
-        s->host4.resize((size_t)npix);
-        s->host1.resize((size_t)npix);
-        HIP_TRY(hipMemcpyAsync(s->host4.data(), s->plane2.p, (size_t)npix * sizeof(float4), hipMemcpyDeviceToHost, st));
-        HIP_TRY(hipMemcpyAsync(s->host1.data(), s->cost.p, (size_t)npix * sizeof(float), hipMemcpyDeviceToHost, st));
-        HIP_TRY(hipStreamSynchronize(st));
-        std::vector<BatchItem> iv(items, items + n);
-        unpack(s->host4, iv, out);
-        for (uint32_t i = 0; i < n; ++i)
-            if (out[i].cost) std::memcpy(out[i].cost, s->host1.data() + items[i].off, (size_t)items[i].w * items[i].h * sizeof(float));
-    }
-    if (raw_depth) {
-        std::vector<float> hd((size_t)npix);
-        HIP_TRY(hipMemcpyAsync(hd.data(), s->snap.p, (size_t)npix * sizeof(float), hipMemcpyDeviceToHost, st));
-        HIP_TRY(hipStreamSynchronize(st));
-        for (uint32_t i = 0; i < n; ++i)
-            if (raw_depth[i]) std::memcpy(raw_depth[i], hd.data() + items[i].off, (size_t)items[i].w * items[i].h * sizeof(float));
-    }
-    if (counts) {
-        std::vector<uint8_t> hc((size_t)npix);
-        HIP_TRY(hipMemcpyAsync(hc.data(), s->count.p, (size_t)npix, hipMemcpyDeviceToHost, st));
-        HIP_TRY(hipStreamSynchronize(st));
-        for (uint32_t i = 0; i < n; ++i)
-            if (counts[i]) std::memcpy(counts[i], hc.data() + items[i].off, (size_t)items[i].w * items[i].h);
-    }
+    rc = download_results(s, items, n, npix, out, counts, raw_depth);
+    if (rc) return rc;
     return C3D_OK;
 }
 
@@ -1455,32 +1443,40 @@ int c3d_pm_densify_learned(c3d_pm_session* s, c3d_init_net* net, const uint32_t*
     quarter.release();
     rc = run_post(s, ls->post, ls->post_k, items.data(), n, timing);
     if (rc) return rc;
-    if (out) {
-        s->host4.resize((size_t)npix);
-        s->host1.resize((size_t)npix);
-        HIP_TRY(hipMemcpyAsync(s->host4.data(), s->plane2.p, (size_t)npix * sizeof(float4), hipMemcpyDeviceToHost, st));
-        HIP_TRY(hipMemcpyAsync(s->host1.data(), s->cost.p, (size_t)npix * sizeof(float), hipMemcpyDeviceToHost, st));
-        HIP_TRY(hipStreamSynchronize(st));
-        unpack(s->host4, items, out);
-        for (uint32_t i = 0; i < n; ++i)
-            if (out[i].cost) std::memcpy(out[i].cost, s->host1.data() + items[i].off, (size_t)items[i].w * items[i].h * sizeof(float));
-    }
-    if (raw_depth) {
-        std::vector<float> hd((size_t)npix);
-        HIP_TRY(hipMemcpyAsync(hd.data(), s->snap.p, (size_t)npix * sizeof(float), hipMemcpyDeviceToHost, st));
-        HIP_TRY(hipStreamSynchronize(st));
-        for (uint32_t i = 0; i < n; ++i)
-            if (raw_depth[i]) std::memcpy(raw_depth[i], hd.data() + items[i].off, (size_t)items[i].w * items[i].h * sizeof(float));
-    }
-    if (counts) {
-        std::vector<uint8_t> hc((size_t)npix);
-        HIP_TRY(hipMemcpyAsync(hc.data(), s->count.p, (size_t)npix, hipMemcpyDeviceToHost, st));
-        HIP_TRY(hipStreamSynchronize(st));
-        for (uint32_t i = 0; i < n; ++i)
-            if (counts[i]) std::memcpy(counts[i], hc.data() + items[i].off, (size_t)items[i].w * items[i].h);
-    }
+    rc = download_results(s, items.data(), n, npix, out, counts, raw_depth);
+    if (rc) return rc;
     return C3D_OK;
 }
+
+}  // extern "C"
+
+namespace {
+int download_results(c3d_pm_session* s, const BatchItem* items, uint32_t n, long long npix, c3d_view_state* out,
+                     uint8_t* const* counts, float* const* raw_depth) {
+    hipStream_t st = s->be->stream;
+    if (out) {
+        HIP_TRY(s->dl_depth.reserve((size_t)npix));
+        HIP_TRY(s->dl_normal.reserve(3 * (size_t)npix));
+        HIP_TRY(launch_split_planes(s->plane2.p, s->dl_depth.p, s->dl_normal.p, npix, st));
+    }
+    for (uint32_t i = 0; i < n; ++i) {
+        const size_t np = (size_t)items[i].w * items[i].h;
+        const long long off = items[i].off;
+        if (out) {
+            HIP_TRY(hipMemcpyAsync(out[i].depth, s->dl_depth.p + off, np * sizeof(float), hipMemcpyDeviceToHost, st));
+            HIP_TRY(hipMemcpyAsync(out[i].normal, s->dl_normal.p + 3 * off, 3 * np * sizeof(float), hipMemcpyDeviceToHost, st));
+            if (out[i].cost) HIP_TRY(hipMemcpyAsync(out[i].cost, s->cost.p + off, np * sizeof(float), hipMemcpyDeviceToHost, st));
+        }
+        if (raw_depth && raw_depth[i])
+            HIP_TRY(hipMemcpyAsync(raw_depth[i], s->snap.p + off, np * sizeof(float), hipMemcpyDeviceToHost, st));
+        if (counts && counts[i]) HIP_TRY(hipMemcpyAsync(counts[i], s->count.p + off, np, hipMemcpyDeviceToHost, st));
+    }
+    HIP_TRY(hipStreamSynchronize(st));
+    return C3D_OK;
+}
+}  // namespace
+
+extern "C" {
 
 int c3d_pm_autotune(c3d_pm_session* s, const uint32_t* views, uint32_t n, uint32_t level, uint32_t* rows_out, uint32_t* tpp_out,
                     double* ms_out) {
