@@ -327,6 +327,25 @@ __device__ __forceinline__ bool homography(const LevelDesc& R, const LevelPair& 
     return true;
 }
 
+// Bilinear interpolation of one quad entry q = p00 | p10 << 8 | p01 << 16 | p11 << 24 at the
+// fractions (ax, ay): top = p00 + ax (p10 - p00), bot = p01 + ax (p11 - p01), top + ay (bot - top),
+// each an FP32 FMA. The two row starts and the two row differences are formed as packed half
+// floats: a byte permute sets the exponent byte 0x64 above each texel, which reads 1024 + p
+// exactly, and one packed subtraction each gives (p00, p01) and (p10 - p00, p11 - p01), all
+// integers of magnitude below 2^11 and so exact. Each row is then one mixed FMA on the exact
+// half-float operands: the same value as the FP32 formula, in 8 instructions instead of 10.
+typedef _Float16 half2_t __attribute__((ext_vector_type(2)));
+__device__ __forceinline__ float quad_lerp(uint32_t q, float ax, float ay) {
+    const uint32_t lo = __builtin_amdgcn_perm(0x64646464u, q, 0x04020400u);  // (1024 + p00, 1024 + p01)
+    const uint32_t hi = __builtin_amdgcn_perm(0x64646464u, q, 0x04030401u);  // (1024 + p10, 1024 + p11)
+    const half2_t l = __builtin_bit_cast(half2_t, lo), h = __builtin_bit_cast(half2_t, hi);
+    const half2_t d = h - l;
+    const half2_t p = l - (half2_t){(_Float16)1024.0f, (_Float16)1024.0f};
+    const float top = fmaf(ax, (float)d.x, (float)p.x);
+    const float bot = fmaf(ax, (float)d.y, (float)p.y);
+    return fmaf(ay, bot - top, top);
+}
+
 // One source sample in gray levels through the quad table (see the layout note above). `qbase` is
 // the table's first entry (x0, y0) = (-2, -2); the byte offset 4 ((y0 + 2) qpitch + (x0 + 2)) is formed
 // exactly in FP32 (multiples of 4 below 2^26) from the clamped floor of the sample position, and the
@@ -338,11 +357,7 @@ __device__ __forceinline__ float quad_sample(const uint32_t* qbase, float qp4, f
     float yc = __builtin_amdgcn_fmed3f(yf, -2.0f, h1);
     uint32_t off = (uint32_t)fmaf(yc, qp4, fmaf(xc, 4.0f, koff4));
     uint32_t q = *(const uint32_t*)((const char*)qbase + off);
-    float p00 = (float)(q & 255u), p10 = (float)((q >> 8) & 255u);
-    float p01 = (float)((q >> 16) & 255u), p11 = (float)(q >> 24);
-    float top = fmaf(ax, p10 - p00, p00);
-    float bot = fmaf(ax, p11 - p01, p01);
-    return fmaf(ay, bot - top, top);
+    return quad_lerp(q, ax, ay);
 }
 
 // Gathers kept in flight by the window loop: this default serves the evaluation, read-out and
@@ -394,11 +409,7 @@ __device__ __forceinline__ void window_sums(const RefWin<WR, WSTEP>& rw, const R
         const int a = k / WN, b = k % WN;
         const uint32_t q = qv[k];
         const float ax = fx[k], ay = fy[k];
-        float p00 = (float)(q & 255u), p10 = (float)((q >> 8) & 255u);
-        float p01 = (float)((q >> 16) & 255u), p11 = (float)(q >> 24);
-        float top = fmaf(ax, p10 - p00, p00);
-        float bot = fmaf(ax, p11 - p01, p01);
-        float sv = fmaf(ay, bot - top, top);
+        float sv = quad_lerp(q, ax, ay);
         if (CHECK) sv = fmaf(H[6], (float)(-WR + b * WSTEP), fmaf(H[7], (float)(-WR + a * WSTEP), H[8])) > 1e-8f ? sv : 0.0f;
         float sp = sv - rw.ic;
         float ws = rw.w[k] * sp;
@@ -567,11 +578,7 @@ __device__ float ncc_cost_dyn(const KParams& P, const RefWinDyn& rw, const REF& 
                 const int dxi = -wr + b * WSTEP;
                 const uint32_t q = qs[u];
                 const float ax = fxs[u], ay = fys[u];
-                float p00 = (float)(q & 255u), p10 = (float)((q >> 8) & 255u);
-                float p01 = (float)((q >> 16) & 255u), p11 = (float)(q >> 24);
-                float top = fmaf(ax, p10 - p00, p00);
-                float bot = fmaf(ax, p11 - p01, p01);
-                float sv = fmaf(ay, bot - top, top);
+                float sv = quad_lerp(q, ax, ay);
                 sv = zs[u] > 1e-8f ? sv : 0.0f;
                 const float iv = ref.at(x + dxi, y + dyi);
                 const float sp = sv - rw.ic;

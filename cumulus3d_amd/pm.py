@@ -42,7 +42,9 @@ class PmError(RuntimeError):
 
 
 def library_path() -> str:
-    return os.path.join(_LIB_DIR, "libc3d_pm.so")
+    """The engine library: $C3D_PM_LIBRARY when set (another build of the same ABI), else the
+    in-tree build."""
+    return os.environ.get("C3D_PM_LIBRARY") or os.path.join(_LIB_DIR, "libc3d_pm.so")
 
 
 def load() -> C.CDLL:
