@@ -74,6 +74,12 @@ def log(msg):
     print(f"[bench] {msg}", file=sys.stderr, flush=True)
 
 
+def shapes_text(shape_ms):
+    """The tile autotuner's timings, one per shape of abi.TILE_SHAPES."""
+    from cumulus3d_amd import abi
+    return ", ".join(f"({r},{t}) {ms:.1f} ms" for (r, t), ms in zip(abi.TILE_SHAPES, shape_ms))
+
+
 def device_sync():
     try:
         hip = C.CDLL("libamdhip64.so")
@@ -328,11 +334,10 @@ def zone_main(args):
                            geometric_iters=args.geometric_iters)
     calib = views[: min(12, nviews)]
     s = be.begin(inp)
-    shape = (4, 1)
+    shape = (1, 1)  # the engine's default shape
     if args.tile_autotune:
         shape, shape_ms = s.autotune(calib, level=0)
-        log(f"tile autotuner on {len(calib)} views: (rows, lanes per pixel) {shape}; (2,1) {shape_ms[0]:.1f} ms, "
-            f"(4,1) {shape_ms[1]:.1f} ms, (4,2) {shape_ms[2]:.1f} ms")
+        log(f"tile autotuner on {len(calib)} views: (rows, lanes per pixel) {shape}; " + shapes_text(shape_ms))
     gate = 0.99
     line_extra = {}
     if args.learned_only:
@@ -541,12 +546,11 @@ def plain_main(args):
     be = pm.HipPatchMatch(0, opts)
     sess = be.begin(inp)
     log(f"session open on {be.name()}, graph {use_graph}")
-    shape = (4, 1)
+    shape = (1, 1)  # the engine's default shape
     if args.tile_autotune:
         # the half step's workgroup shape (rows, lanes per pixel), outside the timed region
         shape, shape_ms = sess.autotune(views, level=inp.num_levels - 1)  # the finest scale dominates
-        log(f"tile autotuner: (rows, lanes per pixel) {shape}; two iterations (2,1) {shape_ms[0]:.1f} ms, "
-            f"(4,1) {shape_ms[1]:.1f} ms, (4,2) {shape_ms[2]:.1f} ms")
+        log(f"tile autotuner: (rows, lanes per pixel) {shape}; two iterations " + shapes_text(shape_ms))
     for i in range(args.warmup):
         sess.densify(views, sched, download=False)
         log(f"warmup {i + 1}/{args.warmup} done")
@@ -658,8 +662,11 @@ def parse_args(argv=None):
     ap.add_argument("--geometric-rounds", type=int, default=1)
     ap.add_argument("--geometric-iters", type=int, default=2)
     ap.add_argument("--graph", type=int, default=None, help="1 = replay the schedule as one hipGraph (default: batched48, fast960)")
-    ap.add_argument("--tile-autotune", type=int, default=1,
-                    help="zone7/batched48: pick the half step's workgroup shape with c3d_pm_autotune before timing (1) or keep (4, 1) (0)")
+    ap.add_argument("--tile-autotune", type=int, default=None,
+                    help="pick the half step's workgroup shape with c3d_pm_autotune before timing (1) or keep the "
+                         "engine's default (1, 1) (0); default 1, 0 for zone108 (the autotuner times iterations from a "
+                         "random start, where the shapes tie, not the learned job's refinement, where one-row "
+                         "workgroups are faster)")
     ap.add_argument("--plain-steps", type=int, default=2, help="zone108: timed jobs of the plain and random-init arms")
     ap.add_argument("--learned-only", action="store_true", help="zone108: time the learned arm only (profiling runs)")
     ap.add_argument("--learned-schedule", type=float, nargs=4, default=None, metavar=("REFINE", "GROUNDS", "GITERS", "CONF"),
@@ -671,7 +678,10 @@ def parse_args(argv=None):
     ap.add_argument("--no-cpu", action="store_true")
     ap.add_argument("--calibrate-copy", type=int, default=0, metavar="MIB",
                     help="also run a device-to-device copy of this many MiB (profiling calibration)")
-    return ap.parse_args(argv)
+    args = ap.parse_args(argv)
+    if args.tile_autotune is None:
+        args.tile_autotune = 0 if args.workload == "zone108" else 1
+    return args
 
 
 def main():

@@ -15,6 +15,9 @@ from typing import List, Optional, Sequence
 import numpy as np
 
 MAX_SOURCES = 32
+# the half step's compiled workgroup shapes (rows of active pixels, lanes per pixel), in the order of
+# c3d_pm_autotune's timings (C3D_TILE_SHAPES)
+TILE_SHAPES = ((1, 1), (2, 1), (4, 1), (4, 2))
 
 C3D_OK = 0
 C3D_INVALID_ARGUMENT = 1

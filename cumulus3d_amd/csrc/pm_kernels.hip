@@ -3,7 +3,7 @@
 //
 // The kernels follow the rules of crates/dense/src/kernel.rs:36-74 as implemented by the reference's
 // crates/cuda/src/kernels/patchmatch.cu (cited per kernel). Work decomposition (MI355X-specific):
-//   * wave64 = 64 active pixels of one row (red-black: every other pixel), 2 or 4 rows per
+//   * wave64 = 64 active pixels of one row (red-black: every other pixel), 1, 2 or 4 rows per
 //     workgroup (autotuned); a workgroup stages its reference patch rows (128 + 2R px wide) in LDS once;
 //   * all views of a batch share one flattened 1-D grid of tiles (no empty blocks for smaller
 //     views), remapped so that consecutive tiles run on the same XCD and share its L2;
@@ -14,8 +14,11 @@
 
 namespace c3d {
 
-// rows of active pixels per half-step workgroup: a compile-time variant (2 or 4), picked per session
-// by the tile autotuner (c3d_pm_autotune)
+// rows of active pixels per half-step workgroup: a compile-time variant (1, 2 or 4), picked per
+// session by the tile autotuner (c3d_pm_autotune). One row (a one-wave workgroup) is the default: a
+// wave's slot on its SIMD is refilled as soon as that wave ends, where a 4-wave workgroup holds its
+// slots (and its LDS) until its slowest row is done, and the rows' work differs through the
+// data-dependent early exits.
 constexpr int HALF_SPAN = 128;   // pixel span of one row of 64 active pixels
 constexpr int PIX_TW = 64;       // eval/filter tiles: 64 x 4 pixels
 constexpr int PIX_TH = 4;
@@ -811,7 +814,7 @@ __global__ void k_pyramid_step(const uint8_t* src, int w, int h, uint8_t* dst, f
 
 // ================================================================ launchers
 
-bool half_shape_ok(int rows, int tpp) { return (rows == 2 && tpp == 1) || (rows == 4 && (tpp == 1 || tpp == 2)); }
+bool half_shape_ok(int rows, int tpp) { return ((rows == 1 || rows == 2) && tpp == 1) || (rows == 4 && (tpp == 1 || tpp == 2)); }
 
 int half_tiles(const BatchItem* items, int n, int rows, int tpp) {
     const int span = HALF_SPAN / tpp;
@@ -836,7 +839,7 @@ static void launch_half_rows(const KParams& P, size_t lds, hipStream_t s) {
 }
 
 // P.ntiles must count the tiles of the shape (half_tiles(..., rows, tpp)); the compiled shapes are
-// (rows, lanes per pixel) = (2, 1), (4, 1) and (4, 2)
+// (rows, lanes per pixel) = (1, 1), (2, 1), (4, 1) and (4, 2)
 template <int WR, int WSTEP>
 static hipError_t launch_half(const KParams& P0, int rows, int tpp, hipStream_t s) {
     if (!half_shape_ok(rows, tpp)) return hipErrorInvalidValue;
@@ -844,7 +847,8 @@ static hipError_t launch_half(const KParams& P0, int rows, int tpp, hipStream_t 
     P.wv_stride = rows * 64;
     size_t lds = (size_t)P.wv_stride * (P.max_nsrc > 0 ? P.max_nsrc : 1) * sizeof(float);
     if (WR == 0) lds += half_tile_bytes(P.wr, rows, tpp);
-    if (rows == 2) launch_half_rows<WR, WSTEP, 2, 1>(P, lds, s);
+    if (rows == 1) launch_half_rows<WR, WSTEP, 1, 1>(P, lds, s);
+    else if (rows == 2) launch_half_rows<WR, WSTEP, 2, 1>(P, lds, s);
     else if (tpp == 1) launch_half_rows<WR, WSTEP, 4, 1>(P, lds, s);
     else launch_half_rows<WR, WSTEP, 4, 2>(P, lds, s);
     return hipGetLastError();

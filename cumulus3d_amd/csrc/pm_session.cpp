@@ -209,7 +209,7 @@ struct c3d_pm_session {
     std::vector<uint8_t> h_need;
     // half-step workgroup shape: rows of active pixels and lanes per pixel (options.tile_rows and
     // options.threads_per_pixel, or c3d_pm_autotune)
-    int half_rows = 4;
+    int half_rows = 1;
     int half_tpp = 1;
     // the captured schedule (options.use_graph) and the key it was captured for
     hipGraphExec_t gexec = nullptr;
@@ -435,6 +435,10 @@ int launch_run(c3d_pm_session* s, KParams k, const std::vector<BatchItem>& items
     return C3D_OK;
 }
 
+// The half step's rows per workgroup for options.tile_rows = 0: one row (one-wave workgroups), or
+// four with two lanes per pixel, the only two-lane shape.
+int default_rows(int rows, int tpp) { return rows != 0 ? rows : (tpp == 2 ? 4 : 1); }
+
 // Sum of the recorded half-step launch times (after the stream has drained).
 int pool_sum(c3d_pm_session* s, double* ms_out, uint32_t* n_out) {
     double tot = 0.0;
@@ -501,10 +505,11 @@ int c3d_pm_create(int ordinal, const c3d_pm_options* opts, c3d_pm_backend** out)
     if (opts) be->opts = *opts;
     {
         const int rows = be->opts.tile_rows, tpp = be->opts.threads_per_pixel;
-        if ((rows != 0 && rows != 2 && rows != 4) || (tpp != 0 && tpp != 1 && tpp != 2) ||
-            !half_shape_ok(rows == 2 ? 2 : 4, tpp == 2 ? 2 : 1)) {
+        if ((rows != 0 && rows != 1 && rows != 2 && rows != 4) || (tpp != 0 && tpp != 1 && tpp != 2) ||
+            !half_shape_ok(default_rows(rows, tpp), tpp == 2 ? 2 : 1)) {
             delete be;
-            return fail(C3D_INVALID_ARGUMENT, "half-step shapes are (tile_rows, threads_per_pixel) = (2, 1), (4, 1), (4, 2)");
+            return fail(C3D_INVALID_ARGUMENT,
+                        "half-step shapes are (tile_rows, threads_per_pixel) = (1, 1), (2, 1), (4, 1), (4, 2)");
         }
     }
     if (be->opts.batch_pixels <= 0) be->opts.batch_pixels = 1LL << 40;
@@ -688,8 +693,8 @@ int c3d_pm_begin(c3d_pm_backend* be, const c3d_kernel_input* in, c3d_pm_session*
     k.filter_gmax = fl.geom_max_cost;
     s->pm_copy = pm;
     s->filter_copy = fl;
-    s->half_rows = be->opts.tile_rows == 2 ? 2 : 4;
     s->half_tpp = be->opts.threads_per_pixel == 2 ? 2 : 1;
+    s->half_rows = default_rows(be->opts.tile_rows, s->half_tpp);
     *out = s;
     return C3D_OK;
 }
@@ -1498,10 +1503,10 @@ int c3d_pm_autotune(c3d_pm_session* s, const uint32_t* views, uint32_t n, uint32
     k.random_init = 1;
     k.ntiles = pix_tiles(items.data(), (int)items.size());
     HIP_TRY(s->ks->eval(k, 1, st));
-    const int cand[3][2] = {{2, 1}, {4, 1}, {4, 2}};
-    double ms[3] = {0.0, 0.0, 0.0};
+    const int cand[C3D_TILE_SHAPES][2] = {{1, 1}, {2, 1}, {4, 1}, {4, 2}};
+    double ms[C3D_TILE_SHAPES] = {};
     int bestc = 0;
-    for (int c = 0; c < 3; ++c) {
+    for (int c = 0; c < C3D_TILE_SHAPES; ++c) {
         KParams h = k;
         h.random_init = 0;
         h.ntiles = half_tiles(items.data(), (int)items.size(), cand[c][0], cand[c][1]);
@@ -1525,7 +1530,7 @@ int c3d_pm_autotune(c3d_pm_session* s, const uint32_t* views, uint32_t n, uint32
     if (rows_out) *rows_out = (uint32_t)s->half_rows;
     if (tpp_out) *tpp_out = (uint32_t)s->half_tpp;
     if (ms_out)
-        for (int c = 0; c < 3; ++c) ms_out[c] = ms[c];
+        for (int c = 0; c < C3D_TILE_SHAPES; ++c) ms_out[c] = ms[c];
     return C3D_OK;
 }
 

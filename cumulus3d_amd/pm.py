@@ -304,11 +304,11 @@ class Session:
 
     def autotune(self, views: Sequence[int], level: int = 0):
         """c3d_pm_autotune: the half-step shape kept for this session, (rows, lanes per pixel), and
-        the timings of (2, 1), (4, 1), (4, 2)."""
+        the timings of the shapes of abi.TILE_SHAPES in that order."""
         rows, tpp = C.c_uint32(), C.c_uint32()
-        ms = (C.c_double * 3)()
+        ms = (C.c_double * len(abi.TILE_SHAPES))()
         _check(self.L, self.L.c3d_pm_autotune(self.h, abi.views_array(views), len(views), level, C.byref(rows), C.byref(tpp), ms))
-        return (int(rows.value), int(tpp.value)), (ms[0], ms[1], ms[2])
+        return (int(rows.value), int(tpp.value)), tuple(ms)
 
     def pair_costs(self, level: int, view: int, slot: int, xy: np.ndarray, planes: np.ndarray) -> np.ndarray:
         """Per-hypothesis photometric cost of explicit planes (c3d_pm_pair_costs)."""

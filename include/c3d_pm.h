@@ -32,7 +32,7 @@
 extern "C" {
 #endif
 
-#define C3D_PM_ABI_VERSION 1
+#define C3D_PM_ABI_VERSION 2
 /* Maximum number of source views per reference view (MAXSRC, patchmatch.rs:52). */
 #define C3D_MAX_SOURCES 32
 
@@ -55,7 +55,8 @@ typedef struct c3d_pm_session c3d_pm_session;
 typedef struct c3d_pm_options {
     int64_t batch_pixels;     /* pixel budget of one launch [2,200,000] (batch_pixels) */
     int32_t max_batch;        /* views per launch [64] (max_batch) */
-    int32_t tile_rows;        /* rows of active pixels per half-step workgroup (2 or 4); 0 = 4 until c3d_pm_autotune */
+    int32_t tile_rows;        /* rows of active pixels per half-step workgroup (1, 2 or 4); 0 = 1 (4 with
+                                 threads_per_pixel 2) until c3d_pm_autotune */
     int32_t threads_per_pixel;/* lanes per pixel in the half step: 1, or 2 with tile_rows 4 (the two lanes
                                  split the pixel's NCC windows); 0 = 1 until c3d_pm_autotune */
     int32_t use_graph;        /* 1 = capture each run's iteration loop in a hipGraph; 0 = plain launches */
@@ -360,12 +361,14 @@ int c3d_fill_holes_noalloc(c3d_pm_backend* backend, float* depth, float* normal,
 
 /* ---------------------------------------------------------------- autotuner (no reference counterpart) */
 
-/* Tile autotuner: times the red-black half step with each compiled workgroup shape — (rows of
- * active pixels, lanes per pixel) = (2, 1), (4, 1), (4, 2); 64 / lanes active pixels per wave row —
- * on `views` at `level` (a random initialisation and two timed photometric iterations on the
- * session's scratch states, after one untimed) and keeps the fastest for the session's later runs.
- * Every shape gives identical results. rows_out / tpp_out (optional) receive the choice, ms_out[3]
- * (optional) the times of the two iterations per shape in that order. */
+/* Tile autotuner: times the red-black half step with each of the C3D_TILE_SHAPES compiled workgroup
+ * shapes — (rows of active pixels, lanes per pixel) = (1, 1), (2, 1), (4, 1), (4, 2); 64 / lanes
+ * active pixels per wave row — on `views` at `level` (a random initialisation and two timed
+ * photometric iterations on the session's scratch states, after one untimed) and keeps the fastest
+ * for the session's later runs. Every shape gives identical results. rows_out / tpp_out (optional)
+ * receive the choice, ms_out[C3D_TILE_SHAPES] (optional) the times of the two iterations per shape
+ * in that order. */
+#define C3D_TILE_SHAPES 4
 int c3d_pm_autotune(c3d_pm_session* session, const uint32_t* views, uint32_t num_views, uint32_t level,
                     uint32_t* rows_out, uint32_t* tpp_out, double* ms_out);
 

@@ -23,3 +23,12 @@ def test_options_parse():
     assert r.returncode == 0
     for opt in ("--steps", "--warmup", "--workload", "--window", "--tile-autotune", "--no-cpu"):
         assert opt in r.stdout
+
+
+def test_tile_autotune_default_depends_on_the_workload():
+    sys.path.insert(0, ROOT)
+    import bench
+
+    assert bench.parse_args([]).tile_autotune == 0  # zone108 keeps the engine's one-row default
+    assert bench.parse_args(["--workload", "zone7"]).tile_autotune == 1
+    assert bench.parse_args(["--tile-autotune", "1"]).tile_autotune == 1

@@ -339,7 +339,7 @@ def test_many_sources_bit_exact_every_tile_shape(cams, size):
 
     o = oracle.Oracle(inp, threads=16)
     ref, rc = run(o)
-    for rows, tpp in ((2, 1), (4, 1), (4, 2)):
+    for rows, tpp in abi.TILE_SHAPES:
         opts = abi.c3d_pm_options()
         opts.tile_rows, opts.threads_per_pixel = rows, tpp
         be = pm.HipPatchMatch(0, opts)

@@ -114,12 +114,12 @@ def test_quality_window_costs_match_fp64():
 
 
 def test_tile_rows_identical_and_autotune():
-    """Half-step workgroups of 2 and 4 rows, and 4 rows with two lanes per pixel (register and
+    """Half-step workgroups of 1, 2 and 4 rows, and 4 rows with two lanes per pixel (register and
     any-radius windows), give identical states and read-out counts (photometric and geometric
-    runs); the autotuner keeps one of the shapes."""
+    runs); the autotuner times every shape and keeps one."""
     sc, inp = _input(192, 108, 5, 2)
     outs = []
-    for rows, tpp, anyr in ((2, 1, 0), (4, 1, 0), (4, 2, 0), (4, 2, 1)):
+    for rows, tpp, anyr in ((1, 1, 0), (2, 1, 0), (4, 1, 0), (4, 2, 0), (4, 2, 1), (1, 1, 1)):
         opts = abi.c3d_pm_options()
         opts.tile_rows = rows
         opts.threads_per_pixel = tpp
@@ -133,7 +133,7 @@ def test_tile_rows_identical_and_autotune():
     with pm.HipPatchMatch().begin(inp) as s:
         shape, ms = s.autotune([0, 1, 2])
     print(f"autotune: (rows, lanes per pixel) {shape}, ms {ms}")
-    assert shape in ((2, 1), (4, 1), (4, 2)) and min(ms) > 0
+    assert shape in abi.TILE_SHAPES and len(ms) == len(abi.TILE_SHAPES) and min(ms) > 0
     bad = abi.c3d_pm_options()
     bad.tile_rows, bad.threads_per_pixel = 2, 2
     with pytest.raises(pm.PmError):
