@@ -613,19 +613,38 @@ __device__ __forceinline__ float ncc_cost_any(const KParams& P, const RefWinT<WR
 }
 
 // Forward-backward reprojection error through the source depth snapshot (patchmatch.cu:292-314).
-__device__ __forceinline__ float geo_error(const KParams& P, const LevelDesc& R, const LevelDesc& S, const PairRec& pr, int x, int y,
-                                           float d) {
+// geo_error in two halves: geo_fetch projects the pixel at depth d into the source and issues the
+// load of the source's snapshot depth at the nearest pixel; geo_finish back-projects it and
+// returns the reprojection distance. A caller can put other work between the two, so that the
+// load's latency is hidden; geo_finish(geo_fetch(...)) is geo_error exactly.
+struct GeoFetch {
+    float u, v, dj;
+    bool ok;  // false: the result is INF_ERR
+};
+
+__device__ __forceinline__ GeoFetch geo_fetch(const KParams& P, const LevelDesc& R, const LevelDesc& S, const PairRec& pr, int x,
+                                              int y, float d) {
+    GeoFetch g{0.f, 0.f, 0.f, false};
     long long off = P.snap_off[pr.src];
-    if (off < 0) return INF_ERR;
+    if (off < 0) return g;
     f3 X = mk3(((float)x - R.cx) / R.fx * d, ((float)y - R.cy) / R.fy * d, d);
     float xj = pr.r[0] * X.x + pr.r[1] * X.y + pr.r[2] * X.z + pr.t[0];
     float yj = pr.r[3] * X.x + pr.r[4] * X.y + pr.r[5] * X.z + pr.t[1];
     float zj = pr.r[6] * X.x + pr.r[7] * X.y + pr.r[8] * X.z + pr.t[2];
-    if (!(zj > 1e-8f)) return INF_ERR;
+    if (!(zj > 1e-8f)) return g;
     float u = S.fx * xj / zj + S.cx, v = S.fy * yj / zj + S.cy;
     float uf = floorf(u + 0.5f), vf = floorf(v + 0.5f);
-    if (!(uf >= 0.f && vf >= 0.f && uf < (float)S.w && vf < (float)S.h)) return INF_ERR;
-    float dj = P.snap[off + (long long)vf * S.w + (long long)uf];
+    if (!(uf >= 0.f && vf >= 0.f && uf < (float)S.w && vf < (float)S.h)) return g;
+    g.dj = P.snap[off + (long long)vf * S.w + (long long)uf];
+    g.u = u;
+    g.v = v;
+    g.ok = true;
+    return g;
+}
+
+__device__ __forceinline__ float geo_finish(const LevelDesc& R, const LevelDesc& S, const PairRec& pr, int x, int y, const GeoFetch& g) {
+    if (!g.ok) return INF_ERR;
+    const float dj = g.dj, u = g.u, v = g.v;
     if (!(dj > 0.f)) return INF_ERR;
     f3 Yj = mk3((u - S.cx) / S.fx * dj, (v - S.cy) / S.fy * dj, dj);
     f3 q = mk3(Yj.x - pr.t[0], Yj.y - pr.t[1], Yj.z - pr.t[2]);
@@ -635,6 +654,11 @@ __device__ __forceinline__ float geo_error(const KParams& P, const LevelDesc& R,
     if (!(Yz > 1e-8f)) return INF_ERR;
     float ex = R.fx * Yx / Yz + R.cx - (float)x, ey = R.fy * Yy / Yz + R.cy - (float)y;
     return sqrtf(ex * ex + ey * ey);
+}
+
+__device__ __forceinline__ float geo_error(const KParams& P, const LevelDesc& R, const LevelDesc& S, const PairRec& pr, int x, int y,
+                                           float d) {
+    return geo_finish(R, S, pr, x, y, geo_fetch(P, R, S, pr, x, y, d));
 }
 
 // (photometric cost, truncated geometric error) of one hypothesis x one source (patchmatch.cu:316-324)

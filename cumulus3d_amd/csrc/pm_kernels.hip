@@ -63,8 +63,11 @@ __device__ __forceinline__ float pair_cost_h(const KParams& P, const RefWinT<WR,
                                              int y, float4 h, float& geo) {
     float H[9];
     float c = 2.0f;
+    // geometric runs: the snapshot depth is fetched before the NCC window and used after it
+    GeoFetch gf{0.f, 0.f, 0.f, false};
+    if constexpr (GEO) gf = geo_fetch(P, R, S, pr, x, y, h.w);
     if (homography(R, lp, ifx, ify, (float)x, (float)y, h.w, xyz(h), H)) c = ncc_cost_any<WR, WSTEP, PIPE, GEO>(P, rw, ref, S, H, x, y);
-    geo = GEO ? fminf(geo_error(P, R, S, pr, x, y, h.w), P.gmax) : 0.f;
+    geo = GEO ? fminf(geo_finish(R, S, pr, x, y, gf), P.gmax) : 0.f;
     return c;
 }
 
