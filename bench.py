@@ -60,11 +60,13 @@ HBM_PEAK_GBS = 8000.0
 # HBM bytes per half-step launch, from rocprofv3 --pmc FETCH_SIZE x 2.0 + WRITE_SIZE x 1.0 (the
 # factors from a 512 MiB device copy in the same run); FETCH_SIZE counts L2 misses, which the
 # Infinity Cache mostly serves: an upper bound on HBM reads.
-#   zone7: 7 x 1920x1080, 6 sources, 7 + 1x2 iterations, averaged over its 18 launches
+# Keyed by workload and the half step's workgroup shape (rows, lanes per pixel), which changes the
+# fetched bytes; other shapes report null:
+#   zone7, (1, 1): 7 x 1920x1080, 6 sources, 7 + 1x2 iterations, averaged over its 18 launches
 #          (profiles/k_half-zone7-7x1080p-6src.md)
-#   zone108_1_1x1: the 108-view zone's learned job, 1 refinement + 1x1 geometric iterations,
-#          averaged over its 4 launches (profiles/k_half-zone108-learned.md)
-K_HALF_TRAFFIC = {"zone7": 25.995e9, "zone108_1_1x1": 73.396e9}
+#   zone108_1_1x1, (1, 1): the 108-view zone's learned job, 1 refinement + 1x1 geometric
+#          iterations, averaged over its 4 launches (profiles/k_half-zone108-learned.md)
+K_HALF_TRAFFIC = {("zone7", (1, 1)): 43.413e9, ("zone108_1_1x1", (1, 1)): 87.979e9}
 FP32_PEAK_TFLOPS = 157.3
 FLOP_PER_SAMPLE = 28  # projective transform 9, reciprocal 1, bilinear 7, scale 1, accumulations 6 (rounded up)
 WN_FAST = 6  # samples per window side at radius 5 step 2
@@ -403,7 +405,7 @@ def zone_main(args):
     avg_bytes, flop = half_step_work(inp, src, launches)
     half_ms = sum(t.half_kernel_ms for t in tms_l) / max(sum(t.half_launches for t in tms_l), 1)
     sched_key = f"zone{nviews}_{learned_s.refine_iters}_{learned_s.geometric_rounds}x{learned_s.geometric_iters}"
-    roof, roof_valu = roofline_objects("k_half<5,2>", avg_bytes, flop, half_ms, K_HALF_TRAFFIC.get(sched_key))
+    roof, roof_valu = roofline_objects("k_half<5,2>", avg_bytes, flop, half_ms, K_HALF_TRAFFIC.get((sched_key, tuple(shape))))
     last = tms_l[-1]
     line = {
         "metric": METRIC,
@@ -591,7 +593,7 @@ def plain_main(args):
     reg_windows = {(5, 2), (4, 2), (3, 2), (2, 2), (3, 1), (2, 1), (1, 1)}
     kernel_name = (f"k_half<{inp.pm.window_radius},{inp.pm.window_step}>"
                    if (inp.pm.window_radius, inp.pm.window_step) in reg_windows else f"k_half<any radius,{inp.pm.window_step}>")
-    roof, roof_valu = roofline_objects(kernel_name, avg_bytes, flop, half_ms, K_HALF_TRAFFIC["zone7"] if default_zone7 else None)
+    roof, roof_valu = roofline_objects(kernel_name, avg_bytes, flop, half_ms, K_HALF_TRAFFIC.get(("zone7", tuple(shape))) if default_zone7 else None)
     nsrc_desc = f"{len(src[0])} src each" if len({len(s) for s in src}) == 1 else \
         f"{min(len(s) for s in src)}-{max(len(s) for s in src)} src"
     size_desc = f"{sizes[0][1]}x{sizes[0][0]}" if len(set(sizes)) == 1 else \
