@@ -662,15 +662,18 @@ __device__ __forceinline__ float geo_error(const KParams& P, const LevelDesc& R,
 }
 
 // (photometric cost, truncated geometric error) of one hypothesis x one source (patchmatch.cu:316-324)
-template <int WR, int WSTEP, class REF>
+// In geometric runs the snapshot depth is fetched before the NCC window and used after it.
+template <int WR, int WSTEP, int PIPE = WINDOW_PIPE, class REF>
 __device__ __forceinline__ float pair_cost(const KParams& P, const RefWinT<WR, WSTEP>& rw, const REF& ref, const LevelDesc& R,
                                            const LevelDesc& S, const PairRec& pr, const LevelPair& lp, int x, int y, float4 h,
                                            float* geo) {
     float H[9];
     float c = 2.0f;
+    GeoFetch gf{0.f, 0.f, 0.f, false};
+    if (P.geometric) gf = geo_fetch(P, R, S, pr, x, y, h.w);
     if (homography(R, lp, 1.0f / R.fx, 1.0f / R.fy, (float)x, (float)y, h.w, xyz(h), H))
-        c = ncc_cost_any<WR, WSTEP>(P, rw, ref, S, H, x, y);
-    if (P.geometric) *geo = fminf(geo_error(P, R, S, pr, x, y, h.w), P.gmax);
+        c = ncc_cost_any<WR, WSTEP, PIPE>(P, rw, ref, S, H, x, y);
+    if (P.geometric) *geo = fminf(geo_finish(R, S, pr, x, y, gf), P.gmax);
     return c;
 }
 

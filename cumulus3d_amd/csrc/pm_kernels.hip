@@ -23,6 +23,15 @@ constexpr int HALF_SPAN = 128;   // pixel span of one row of 64 active pixels
 constexpr int PIX_TW = 64;       // eval/filter tiles: 64 x 4 pixels
 constexpr int PIX_TH = 4;
 
+// The run-start evaluation and the read-out keep five source gathers in flight through a window
+// and, for register-held windows of at most 36 samples, ask for three waves per SIMD: 156-158
+// VGPRs without spills (eight in flight needed 180-187 VGPRs, two waves per SIMD).
+constexpr int EVAL_PIPE = 5;
+template <int WR, int WSTEP>
+constexpr int eval_waves() {
+    return (WR > 0 && Win<(WR > 0 ? WR : 1), WSTEP>::WN2 <= 36) ? 3 : 1;
+}
+
 // region offsets for the up direction (patchmatch.cu:424-436)
 __constant__ int V_OFF[7][2] = {{0, -1}, {-1, -2}, {1, -2}, {-2, -3}, {2, -3}, {-3, -4}, {3, -4}};
 
@@ -413,7 +422,7 @@ __device__ __forceinline__ float eval_cost(const KParams& P, const ViewDesc& vd,
         const PairRec& pr = prs[j];
         const LevelDesc S = P.lv[pr.src * P.nlevels + P.level];
         float gq = 0.f;
-        float cc = pair_cost<WR, WSTEP>(P, rw, ref, R, S, pr, lps[j], x, y, h, &gq);
+        float cc = pair_cost<WR, WSTEP, EVAL_PIPE>(P, rw, ref, R, S, pr, lps[j], x, y, h, &gq);
         if (P.geometric) cc = cc + P.lambda * gq;
         if (cc < b0) {
             b2 = b1;
@@ -437,7 +446,7 @@ __device__ __forceinline__ float eval_cost(const KParams& P, const ViewDesc& vd,
 // pixels of colour P.color of 128 x 4 spans (one lane per pixel, as the half step), cost only
 // (k_eval_init has initialised them).
 template <int WR, int WSTEP, bool ONE>
-__global__ __launch_bounds__(256) void k_eval(KParams P, int mode) {
+__global__ __launch_bounds__(256) __attribute__((amdgpu_waves_per_eu(eval_waves<WR, WSTEP>()))) void k_eval(KParams P, int mode) {
     constexpr int SPAN = ONE ? HALF_SPAN : PIX_TW;
     constexpr int TW = SPAN + 2 * WR;
     __shared__ float tile_s[WR > 0 ? TW * (PIX_TH + 2 * WR) : 1];
@@ -490,7 +499,7 @@ __global__ __launch_bounds__(256) void k_eval_init(KParams P) {
 
 // ================================================================ final read-out (patchmatch.cu:694-737)
 template <int WR, int WSTEP>
-__global__ __launch_bounds__(256) void k_filter(KParams P) {
+__global__ __launch_bounds__(256) __attribute__((amdgpu_waves_per_eu(eval_waves<WR, WSTEP>()))) void k_filter(KParams P) {
     constexpr int TW = PIX_TW + 2 * WR;
     __shared__ float tile_s[WR > 0 ? TW * (PIX_TH + 2 * WR) : 1];
     extern __shared__ float dyn_lds[];
@@ -531,10 +540,11 @@ __global__ __launch_bounds__(256) void k_filter(KParams P) {
             if (cphi <= 0.f) continue;
             float H[9];
             if (!homography(R, lps[j], 1.0f / R.fx, 1.0f / R.fy, (float)x, (float)y, h.w, xyz(h), H)) continue;
-            float m = ncc_cost_any<WR, WSTEP>(P, rw, ref, S, H, x, y);
+            const GeoFetch gf = geo_fetch(P, R, S, pr, x, y, h.w);  // used after the window
+            float m = ncc_cost_any<WR, WSTEP, EVAL_PIPE>(P, rw, ref, S, H, x, y);
             float e = P.ncc_norm * c3d_expf(-m * m * P.inv2ncc);
             if (e / (e + 0.5f) < P.q_min) continue;
-            if (geo_error(P, R, S, pr, x, y, h.w) > P.filter_gmax) continue;
+            if (geo_finish(R, S, pr, x, y, gf) > P.filter_gmax) continue;
             count++;
         }
     }
