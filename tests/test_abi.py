@@ -121,8 +121,30 @@ def test_no_device_means_not_available():
 
 
 def test_missing_library_fails_loudly(monkeypatch, tmp_path):
+    monkeypatch.delenv("C3D_PM_LIBRARY", raising=False)
     monkeypatch.setattr(pm, "_LIB_DIR", str(tmp_path))
     monkeypatch.setattr(pm, "_libs", {})
     with pytest.raises(pm.PmError) as e:
         pm.load()
     assert e.value.status == abi.C3D_NOT_AVAILABLE
+
+
+def test_library_path_can_name_another_build(monkeypatch, tmp_path):
+    """$C3D_PM_LIBRARY selects another build of the engine; a path that does not exist fails as a
+    missing in-tree build does, naming the path."""
+    import shutil
+
+    monkeypatch.setattr(pm, "_libs", {})
+    monkeypatch.setenv("C3D_PM_LIBRARY", str(tmp_path / "nowhere.so"))
+    with pytest.raises(pm.PmError) as e:
+        pm.load()
+    assert e.value.status == abi.C3D_NOT_AVAILABLE and "nowhere.so" in e.value.message
+    monkeypatch.delenv("C3D_PM_LIBRARY")
+    in_tree = pm.library_path()
+    other = tmp_path / "copy.so"
+    shutil.copy(in_tree, other)
+    monkeypatch.setattr(pm, "_libs", {})
+    monkeypatch.setenv("C3D_PM_LIBRARY", str(other))
+    assert pm.library_path() == str(other)
+    L = pm.load()
+    assert L._name == str(other) and hasattr(L, "c3d_pm_create")
