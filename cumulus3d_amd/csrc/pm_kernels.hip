@@ -483,16 +483,25 @@ __global__ __launch_bounds__(256) __attribute__((amdgpu_waves_per_eu(eval_waves<
 }
 
 // The window-free part of a run start for every pixel: random initialisation by random_init and
-// the best-view reset (patchmatch.cu:668-677). Stored costs are not touched.
-__global__ __launch_bounds__(256) void k_eval_init(KParams P) {
-    BatchItem it;
-    int px0, py0;
-    if (!locate_tile(P, blockIdx.x, PIX_TW, PIX_TH, it, px0, py0)) return;
-    const int x = px0 + (threadIdx.x & 63), y = py0 + (threadIdx.x >> 6);
-    if (x >= it.w || y >= it.h) return;
+// the best-view reset (patchmatch.cu:668-677). Stored costs are not touched. One thread per pixel
+// of the flattened batch, its item found by a binary search over the items' pixel offsets (stored
+// in increasing order): a few loads per thread, where a workgroup's linear search over the items
+// for its tile cost more than the pass itself on a 108-view batch.
+__global__ __launch_bounds__(256) void k_eval_init(KParams P, long long npix) {
+    const long long g = (long long)blockIdx.x * 256 + threadIdx.x;
+    if (g >= npix) return;
+    int lo = 0, hi = P.nitems - 1;
+    while (lo < hi) {
+        const int mid = (lo + hi + 1) >> 1;
+        if (P.batch[mid].off <= g) lo = mid;
+        else hi = mid - 1;
+    }
+    const BatchItem it = P.batch[lo];
+    if (g < it.off || g - it.off >= (long long)it.w * it.h) return;
+    const int pix = (int)(g - it.off);
+    const int y = pix / it.w, x = pix - y * it.w;
     const LevelDesc R = P.lv[it.view * P.nlevels + P.level];
     const ViewDesc vd = P.views[it.view];
-    const int pix = y * it.w + x;
     eval_init_plane(P, vd, R, x, y, pix, it.off, 1);
     P.bview[it.off + pix] = 255;
 }
@@ -944,9 +953,10 @@ hipError_t launch_split_planes(const float4* plane, float* depth, float* normal,
     return hipGetLastError();
 }
 
-// P.ntiles must count 64 x 4 tiles (pix_tiles)
-hipError_t launch_eval_init(const KParams& P, hipStream_t s) {
-    hipLaunchKernelGGL(k_eval_init, dim3(P.ntiles), dim3(256), 0, s, P);
+// npix: the pixels of the batch (the end of its last item)
+hipError_t launch_eval_init(const KParams& P, long long npix, hipStream_t s) {
+    if (npix <= 0) return hipSuccess;
+    hipLaunchKernelGGL(k_eval_init, dim3((unsigned)((npix + 255) / 256)), dim3(256), 0, s, P, npix);
     return hipGetLastError();
 }
 

@@ -406,7 +406,8 @@ int launch_run(c3d_pm_session* s, KParams k, const std::vector<BatchItem>& items
     } else {
         // the colour-0 costs of the run start are overwritten unread by the first half step: only
         // colour 1 is costed (k_eval_init initialises every pixel and resets the best view)
-        HIP_TRY(launch_eval_init(k, st));
+        const long long npix = items.empty() ? 0 : items.back().off + (long long)items.back().w * items.back().h;
+        HIP_TRY(launch_eval_init(k, npix, st));
         KParams e = k;
         e.color = 1;
         e.ntiles = half_tiles(items.data(), (int)items.size(), 4, 1);
