@@ -482,23 +482,33 @@ __global__ __launch_bounds__(256) __attribute__((amdgpu_waves_per_eu(eval_waves<
     if (!ONE && mode == 1) P.bview[base + pix] = 255;
 }
 
-// The window-free part of a run start for every pixel: random initialisation by random_init and
-// the best-view reset (patchmatch.cu:668-677). Stored costs are not touched. One thread per pixel
-// of the flattened batch, its item found by a binary search over the items' pixel offsets (stored
-// in increasing order): a few loads per thread, where a workgroup's linear search over the items
-// for its tile cost more than the pass itself on a 108-view batch.
-__global__ __launch_bounds__(256) void k_eval_init(KParams P, long long npix) {
-    const long long g = (long long)blockIdx.x * 256 + threadIdx.x;
-    if (g >= npix) return;
+// The batch item holding pixel g of the flattened batch, by a binary search over the items' pixel
+// offsets (stored in increasing order); false for a pixel outside every item. Per-pixel passes use
+// it instead of locate_tile: a few loads per thread, where a workgroup's linear search over the
+// items for its tile costs more than a light pass itself on a 108-view batch.
+__device__ __forceinline__ bool locate_pixel(const KParams& P, long long g, BatchItem& it, int& item, int& pix) {
     int lo = 0, hi = P.nitems - 1;
     while (lo < hi) {
         const int mid = (lo + hi + 1) >> 1;
         if (P.batch[mid].off <= g) lo = mid;
         else hi = mid - 1;
     }
-    const BatchItem it = P.batch[lo];
-    if (g < it.off || g - it.off >= (long long)it.w * it.h) return;
-    const int pix = (int)(g - it.off);
+    it = P.batch[lo];
+    item = lo;
+    if (g < it.off || g - it.off >= (long long)it.w * it.h) return false;
+    pix = (int)(g - it.off);
+    return true;
+}
+
+// The window-free part of a run start for every pixel: random initialisation by random_init and
+// the best-view reset (patchmatch.cu:668-677). Stored costs are not touched. One thread per pixel
+// of the flattened batch.
+__global__ __launch_bounds__(256) void k_eval_init(KParams P, long long npix) {
+    const long long g = (long long)blockIdx.x * 256 + threadIdx.x;
+    if (g >= npix) return;
+    BatchItem it;
+    int item, pix;
+    if (!locate_pixel(P, g, it, item, pix)) return;
     const int y = pix / it.w, x = pix - y * it.w;
     const LevelDesc R = P.lv[it.view * P.nlevels + P.level];
     const ViewDesc vd = P.views[it.view];
@@ -776,13 +786,13 @@ __global__ void k_extract_depth(const float4* plane, float* out, long long n) {
 }
 
 // read-out threshold: depth/normal kept where depth > 0 and count >= need (densify.rs:446-454)
-__global__ void k_threshold(KParams P, const uint8_t* need_per_item, float4* out) {
+__global__ __launch_bounds__(256) void k_threshold(KParams P, const uint8_t* need_per_item, float4* out, long long npix) {
+    const long long g = (long long)blockIdx.x * 256 + threadIdx.x;
+    if (g >= npix) return;
     BatchItem it;
-    int px0, py0, item = 0;
-    if (!locate_tile(P, blockIdx.x, PIX_TW, PIX_TH, it, px0, py0, &item)) return;
-    const int x = px0 + (threadIdx.x & 63), y = py0 + (threadIdx.x >> 6);
-    if (x >= it.w || y >= it.h) return;
-    long long p = it.off + y * it.w + x;
+    int item, pix;
+    if (!locate_pixel(P, g, it, item, pix)) return;
+    const long long p = it.off + pix;
     float4 h = P.plane[p];
     if (!(h.w > 0.f && P.count[p] >= need_per_item[item])) h = make_float4(0.f, 0.f, 0.f, 0.f);
     out[p] = h;
@@ -988,8 +998,10 @@ hipError_t launch_pyramid_step(const uint8_t* src, int w, int h, uint8_t* dst, c
     hipLaunchKernelGGL(k_pyramid_step, dim3((n + 255) / 256), dim3(256), 0, s, src, w, h, dst, g[0], g[1], g[2]);
     return hipGetLastError();
 }
-hipError_t launch_threshold(const KParams& P, const uint8_t* need, float4* out, hipStream_t s) {
-    hipLaunchKernelGGL(k_threshold, dim3(P.ntiles), dim3(256), 0, s, P, need, out);
+// npix: the pixels of the batch (the end of its last item)
+hipError_t launch_threshold(const KParams& P, const uint8_t* need, float4* out, long long npix, hipStream_t s) {
+    if (npix <= 0) return hipSuccess;
+    hipLaunchKernelGGL(k_threshold, dim3((unsigned)((npix + 255) / 256)), dim3(256), 0, s, P, need, out, npix);
     return hipGetLastError();
 }
 

@@ -1087,7 +1087,7 @@ int enqueue_densify(c3d_pm_session* s, const DensifyPlan& pl, const c3d_schedule
     KParams f = plan_params(s, pl, pl.top, true);
     f.ntiles = pix_tiles(items, (int)pl.n);
     HIP_TRY(s->ks->filter(f, st));
-    HIP_TRY(launch_threshold(f, s->need.p, s->plane2.p, st));
+    HIP_TRY(launch_threshold(f, s->need.p, s->plane2.p, npix, st));
     if (timed) MARK(-1);
     return C3D_OK;
 }
@@ -1426,7 +1426,7 @@ int c3d_pm_densify_learned(c3d_pm_session* s, c3d_init_net* net, const uint32_t*
     KParams f = params_for(s, top, true, items, max_nsrc);
     f.ntiles = pix_tiles(items.data(), (int)items.size());
     HIP_TRY(s->ks->filter(f, st));
-    HIP_TRY(launch_threshold(f, s->need.p, s->plane2.p, st));
+    HIP_TRY(launch_threshold(f, s->need.p, s->plane2.p, npix, st));
     MARK(-1);
     HIP_TRY(hipEventSynchronize(s->marks[s->marks_used - 1]));
     if (timing) {
