@@ -83,6 +83,9 @@ struct KParams {
     float eps, phi, up_inv2s, up_inv2c;
     float cos_filter_tri, q_min, ncc_norm, inv2ncc;
     float filter_gmax;
+    // first flattened tile of each batch item for the launch's tile shape, tile_pre[nitems] = all
+    // tiles (null: locate_tile searches the items linearly)
+    const int* tile_pre;
     int wv_stride;  // LDS floats per source in the per-thread view-weight table
     int ntiles;     // total workgroups of the launch
     int max_nsrc;   // largest source count of the batch (sizes the LDS view-weight table)
@@ -118,6 +121,26 @@ __device__ __forceinline__ bool locate_tile(const KParams& P, int t, int tw, int
         t -= n;
     }
     return false;
+}
+
+// locate_tile through the launch's tile offsets (P.tile_pre, built for the same tw x th): a binary
+// search over the items instead of a linear one. The last item whose first tile is at or before t
+// holds it (items without tiles share their successor's offset and are passed over).
+__device__ __forceinline__ bool locate_tile_pre(const KParams& P, int t, int tw, int th, BatchItem& it, int& x0, int& y0) {
+    const int* pre = P.tile_pre;
+    if (t >= pre[P.nitems]) return false;
+    int lo = 0, hi = P.nitems - 1;
+    while (lo < hi) {
+        const int mid = (lo + hi + 1) >> 1;
+        if (pre[mid] <= t) lo = mid;
+        else hi = mid - 1;
+    }
+    it = P.batch[lo];
+    const int tl = t - pre[lo];
+    const int tx = (it.w + tw - 1) / tw;
+    y0 = (tl / tx) * th;
+    x0 = (tl % tx) * tw;
+    return true;
 }
 
 // ---------------------------------------------------------------- random numbers (kernel.rs:248-259)

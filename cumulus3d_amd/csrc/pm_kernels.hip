@@ -108,7 +108,8 @@ __global__ __launch_bounds__(ROWS * 64) __attribute__((amdgpu_waves_per_eu(2, 2)
 
     BatchItem it;
     int px0, py0;
-    if (!locate_tile(P, xcd_tile(blockIdx.x, P.ntiles), SPAN, ROWS, it, px0, py0)) return;
+    const int tix = xcd_tile(blockIdx.x, P.ntiles);
+    if (!(P.tile_pre ? locate_tile_pre(P, tix, SPAN, ROWS, it, px0, py0) : locate_tile(P, tix, SPAN, ROWS, it, px0, py0))) return;
     const LevelDesc R = P.lv[it.view * P.nlevels + P.level];
     const int w = it.w, h = it.h;
     {
@@ -998,6 +999,25 @@ hipError_t launch_pyramid_step(const uint8_t* src, int w, int h, uint8_t* dst, c
     hipLaunchKernelGGL(k_pyramid_step, dim3((n + 255) / 256), dim3(256), 0, s, src, w, h, dst, g[0], g[1], g[2]);
     return hipGetLastError();
 }
+// Tile offsets of the batch items for tiles of tw x th pixels: pre[i] = tiles of the items before
+// i, pre[n] = all tiles. One lane sums the counts in item order (a few hundred items at most).
+__global__ void k_tile_prefix(const BatchItem* batch, int n, int tw, int th, int* pre) {
+    if (blockIdx.x != 0 || threadIdx.x != 0) return;
+    int acc = 0;
+    for (int i = 0; i < n; ++i) {
+        const BatchItem b = batch[i];
+        pre[i] = acc;
+        acc += ((b.w + tw - 1) / tw) * ((b.h + th - 1) / th);
+    }
+    pre[n] = acc;
+}
+
+// pre: n + 1 ints; the half step's tiles for (rows, lanes per pixel) are HALF_SPAN / tpp x rows
+hipError_t launch_half_tile_prefix(const BatchItem* batch, int n, int rows, int tpp, int* pre, hipStream_t s) {
+    hipLaunchKernelGGL(k_tile_prefix, dim3(1), dim3(64), 0, s, batch, n, HALF_SPAN / tpp, rows, pre);
+    return hipGetLastError();
+}
+
 // npix: the pixels of the batch (the end of its last item)
 hipError_t launch_threshold(const KParams& P, const uint8_t* need, float4* out, long long npix, hipStream_t s) {
     if (npix <= 0) return hipSuccess;

@@ -184,6 +184,8 @@ struct c3d_pm_session {
     DevBuf<uint8_t> bview, count, need;
     // downloads: the final planes split into depth and 3-float normal arrays
     DevBuf<float> dl_depth, dl_normal;
+    // first half-step tile of each batch item (launch_half_tile_prefix), rebuilt by every run
+    DevBuf<int> tile_pre;
     // probes
     DevBuf<int> q_xy;
     DevBuf<float4> q_planes;
@@ -284,6 +286,7 @@ int set_batch(c3d_pm_session* s, const uint32_t* views, uint32_t a, uint32_t b, 
     HIP_TRY(s->cost.reserve(npix));
     HIP_TRY(s->bview.reserve(npix));
     HIP_TRY(s->count.reserve(npix));
+    HIP_TRY(s->tile_pre.reserve(items.size() + 1));
     return C3D_OK;
 }
 
@@ -303,6 +306,7 @@ KParams params_for(c3d_pm_session* s, uint32_t level, bool geometric, const std:
     k.bview = s->bview.p;
     k.count = s->count.p;
     k.prior = s->prior.p;
+    k.tile_pre = nullptr;
     k.nitems = (int)items.size();
     k.level = (int)level;
     k.geometric = geometric ? 1 : 0;
@@ -414,6 +418,10 @@ int launch_run(c3d_pm_session* s, KParams k, const std::vector<BatchItem>& items
         HIP_TRY(s->ks->eval_color(e, st));
     }
     k.ntiles = half_tiles(items.data(), (int)items.size(), s->half_rows, s->half_tpp);
+    if (iterations > 0) {
+        HIP_TRY(launch_half_tile_prefix(k.batch, k.nitems, s->half_rows, s->half_tpp, s->tile_pre.p, st));
+        k.tile_pre = s->tile_pre.p;
+    }
     for (uint32_t t = 0; t < iterations; ++t) {
         iteration_constants(pm, geometric, t, k);
         for (int color = 0; color < 2; ++color) {
@@ -737,6 +745,7 @@ void c3d_pm_end(c3d_pm_session* s) {
     s->d_snap_lv.release();
     s->dl_depth.release();
     s->dl_normal.release();
+    s->tile_pre.release();
     c3d_pm_backend* be = s->be;
     delete s;  // unlocks the backend's session mutex
     c3d::backend_release(be);
@@ -989,6 +998,7 @@ int plan_densify(c3d_pm_session* s, const uint32_t* views, uint32_t n, const c3d
     HIP_TRY(s->count.reserve(m));
     HIP_TRY(s->snap.reserve(m));
     HIP_TRY(s->plane2.reserve(m));
+    HIP_TRY(s->tile_pre.reserve((size_t)n + 1));
     HIP_TRY(s->need.reserve(n));
     if (s->nl > 1) {
         HIP_TRY(s->low.reserve(m));
@@ -1144,7 +1154,7 @@ std::vector<uint8_t> graph_key(c3d_pm_session* s, const uint32_t* views, uint32_
     put(&s->half_rows, sizeof s->half_rows);
     put(&s->half_tpp, sizeof s->half_tpp);
     const void* ptrs[] = {s->plane.p, s->cost.p, s->bview.p, s->count.p, s->snap.p, s->plane2.p, s->need.p, s->low.p,
-                          s->up.p, s->cinit.p, s->prior.p, s->d_batch_lv.p, s->d_snap_lv.p};
+                          s->up.p, s->cinit.p, s->prior.p, s->d_batch_lv.p, s->d_snap_lv.p, s->tile_pre.p};
     put(ptrs, sizeof ptrs);
     return k;
 }
@@ -1511,6 +1521,8 @@ int c3d_pm_autotune(c3d_pm_session* s, const uint32_t* views, uint32_t n, uint32
         KParams h = k;
         h.random_init = 0;
         h.ntiles = half_tiles(items.data(), (int)items.size(), cand[c][0], cand[c][1]);
+        HIP_TRY(launch_half_tile_prefix(h.batch, h.nitems, cand[c][0], cand[c][1], s->tile_pre.p, st));
+        h.tile_pre = s->tile_pre.p;
         for (int it = 0; it < 3; ++it) {
             iteration_constants(s->pm_copy, false, (uint32_t)it, h);
             if (it == 1) HIP_TRY(hipEventRecord(s->ev[0], st));
