@@ -143,6 +143,12 @@ __device__ __forceinline__ bool locate_tile_pre(const KParams& P, int t, int tw,
     return true;
 }
 
+// The tile's item and pixel origin through P.tile_pre when the launch provides it (built for the
+// same tw x th), else by the linear search.
+__device__ __forceinline__ bool find_tile(const KParams& P, int t, int tw, int th, BatchItem& it, int& x0, int& y0) {
+    return P.tile_pre ? locate_tile_pre(P, t, tw, th, it, x0, y0) : locate_tile(P, t, tw, th, it, x0, y0);
+}
+
 // ---------------------------------------------------------------- random numbers (kernel.rs:248-259)
 
 __device__ __forceinline__ unsigned long long mix64(unsigned long long z) {

@@ -108,8 +108,7 @@ __global__ __launch_bounds__(ROWS * 64) __attribute__((amdgpu_waves_per_eu(2, 2)
 
     BatchItem it;
     int px0, py0;
-    const int tix = xcd_tile(blockIdx.x, P.ntiles);
-    if (!(P.tile_pre ? locate_tile_pre(P, tix, SPAN, ROWS, it, px0, py0) : locate_tile(P, tix, SPAN, ROWS, it, px0, py0))) return;
+    if (!find_tile(P, xcd_tile(blockIdx.x, P.ntiles), SPAN, ROWS, it, px0, py0)) return;
     const LevelDesc R = P.lv[it.view * P.nlevels + P.level];
     const int w = it.w, h = it.h;
     {
@@ -456,7 +455,7 @@ __global__ __launch_bounds__(256) __attribute__((amdgpu_waves_per_eu(eval_waves<
     float* tile = WR > 0 ? tile_s : dyn_lds;
     BatchItem it;
     int px0, py0;
-    if (!locate_tile(P, xcd_tile(blockIdx.x, P.ntiles), SPAN, PIX_TH, it, px0, py0)) return;
+    if (!find_tile(P, xcd_tile(blockIdx.x, P.ntiles), SPAN, PIX_TH, it, px0, py0)) return;
     const LevelDesc R = P.lv[it.view * P.nlevels + P.level];
     {
         const uint8_t* g = P.gray + R.goff;
@@ -527,7 +526,7 @@ __global__ __launch_bounds__(256) __attribute__((amdgpu_waves_per_eu(eval_waves<
     float* tile = WR > 0 ? tile_s : dyn_lds;
     BatchItem it;
     int px0, py0;
-    if (!locate_tile(P, xcd_tile(blockIdx.x, P.ntiles), PIX_TW, PIX_TH, it, px0, py0)) return;
+    if (!find_tile(P, xcd_tile(blockIdx.x, P.ntiles), PIX_TW, PIX_TH, it, px0, py0)) return;
     const LevelDesc R = P.lv[it.view * P.nlevels + P.level];
     stage_pix_tile(tile, P.gray + R.goff, it.w, it.h, px0, py0, wr);
     __syncthreads();
@@ -1015,6 +1014,16 @@ __global__ void k_tile_prefix(const BatchItem* batch, int n, int tw, int th, int
 // pre: n + 1 ints; the half step's tiles for (rows, lanes per pixel) are HALF_SPAN / tpp x rows
 hipError_t launch_half_tile_prefix(const BatchItem* batch, int n, int rows, int tpp, int* pre, hipStream_t s) {
     hipLaunchKernelGGL(k_tile_prefix, dim3(1), dim3(64), 0, s, batch, n, HALF_SPAN / tpp, rows, pre);
+    return hipGetLastError();
+}
+// the 64 x 4 tiles of k_eval (all pixels) and k_filter (pix_tiles)
+hipError_t launch_pix_tile_prefix(const BatchItem* batch, int n, int* pre, hipStream_t s) {
+    hipLaunchKernelGGL(k_tile_prefix, dim3(1), dim3(64), 0, s, batch, n, PIX_TW, PIX_TH, pre);
+    return hipGetLastError();
+}
+// the 128 x 4 spans of k_eval over the pixels of one colour (half_tiles(..., 4, 1))
+hipError_t launch_span_tile_prefix(const BatchItem* batch, int n, int* pre, hipStream_t s) {
+    hipLaunchKernelGGL(k_tile_prefix, dim3(1), dim3(64), 0, s, batch, n, HALF_SPAN, PIX_TH, pre);
     return hipGetLastError();
 }
 

@@ -43,5 +43,7 @@ hipError_t launch_restore(float4* plane, const float4* up, const float* c_init, 
 hipError_t launch_pyramid_step(const uint8_t* src, int w, int h, uint8_t* dst, const float g[3], hipStream_t s);
 hipError_t launch_threshold(const KParams& P, const uint8_t* need, float4* out, long long npix, hipStream_t s);
 hipError_t launch_half_tile_prefix(const BatchItem* batch, int n, int rows, int tpp, int* pre, hipStream_t s);
+hipError_t launch_pix_tile_prefix(const BatchItem* batch, int n, int* pre, hipStream_t s);
+hipError_t launch_span_tile_prefix(const BatchItem* batch, int n, int* pre, hipStream_t s);
 
 }  // namespace c3d

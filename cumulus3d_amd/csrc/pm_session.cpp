@@ -415,6 +415,8 @@ int launch_run(c3d_pm_session* s, KParams k, const std::vector<BatchItem>& items
         KParams e = k;
         e.color = 1;
         e.ntiles = half_tiles(items.data(), (int)items.size(), 4, 1);
+        HIP_TRY(launch_span_tile_prefix(e.batch, e.nitems, s->tile_pre.p, st));
+        e.tile_pre = s->tile_pre.p;
         HIP_TRY(s->ks->eval_color(e, st));
     }
     k.ntiles = half_tiles(items.data(), (int)items.size(), s->half_rows, s->half_tpp);
@@ -1096,6 +1098,8 @@ int enqueue_densify(c3d_pm_session* s, const DensifyPlan& pl, const c3d_schedule
     HIP_TRY(launch_extract_depth(s->plane.p, s->snap.p, npix, st));
     KParams f = plan_params(s, pl, pl.top, true);
     f.ntiles = pix_tiles(items, (int)pl.n);
+    HIP_TRY(launch_pix_tile_prefix(f.batch, f.nitems, s->tile_pre.p, st));
+    f.tile_pre = s->tile_pre.p;
     HIP_TRY(s->ks->filter(f, st));
     HIP_TRY(launch_threshold(f, s->need.p, s->plane2.p, npix, st));
     if (timed) MARK(-1);
@@ -1435,6 +1439,8 @@ int c3d_pm_densify_learned(c3d_pm_session* s, c3d_init_net* net, const uint32_t*
     HIP_TRY(launch_extract_depth(s->plane.p, s->snap.p, npix, st));
     KParams f = params_for(s, top, true, items, max_nsrc);
     f.ntiles = pix_tiles(items.data(), (int)items.size());
+    HIP_TRY(launch_pix_tile_prefix(f.batch, f.nitems, s->tile_pre.p, st));
+    f.tile_pre = s->tile_pre.p;
     HIP_TRY(s->ks->filter(f, st));
     HIP_TRY(launch_threshold(f, s->need.p, s->plane2.p, npix, st));
     MARK(-1);
