@@ -626,7 +626,7 @@ __global__ __launch_bounds__(256) void k_upsample(KParams P) {
     __shared__ float lgray[UP_CH * UP_CW];
     BatchItem it;
     int px0, py0;
-    if (!locate_tile(P, xcd_tile(blockIdx.x, P.ntiles), PIX_TW, PIX_TH, it, px0, py0)) return;  // uniform per workgroup
+    if (!find_tile(P, xcd_tile(blockIdx.x, P.ntiles), PIX_TW, PIX_TH, it, px0, py0)) return;  // uniform per workgroup
     const LevelDesc R = P.lv[it.view * P.nlevels + P.level];
     const LevelDesc L = P.lv[it.view * P.nlevels + P.level - 1];
     const float4* low = P.prior + it.low_off;
@@ -712,7 +712,7 @@ __global__ __launch_bounds__(256) void k_median(KParams P) {
     __shared__ float4 rec[MED_CH * MED_CW];
     BatchItem it;
     int px0, py0;
-    if (!locate_tile(P, xcd_tile(blockIdx.x, P.ntiles), PIX_TW, PIX_TH, it, px0, py0)) return;  // uniform per workgroup
+    if (!find_tile(P, xcd_tile(blockIdx.x, P.ntiles), PIX_TW, PIX_TH, it, px0, py0)) return;  // uniform per workgroup
     const LevelDesc R = P.lv[it.view * P.nlevels + P.level];
     const float4* in = P.prior + it.off;
     const int w = it.w, h = it.h;

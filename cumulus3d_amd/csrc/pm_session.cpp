@@ -406,7 +406,10 @@ int launch_run(c3d_pm_session* s, KParams k, const std::vector<BatchItem>& items
     hipStream_t st = s->be->stream;
     k.ntiles = pix_tiles(items.data(), (int)items.size());
     if (iterations == 0) {
-        HIP_TRY(s->ks->eval(k, 1, st));
+        HIP_TRY(launch_pix_tile_prefix(k.batch, k.nitems, s->tile_pre.p, st));
+        KParams e = k;
+        e.tile_pre = s->tile_pre.p;
+        HIP_TRY(s->ks->eval(e, 1, st));
     } else {
         // the colour-0 costs of the run start are overwritten unread by the first half step: only
         // colour 1 is costed (k_eval_init initialises every pixel and resets the best view)
@@ -1056,6 +1059,8 @@ int enqueue_densify(c3d_pm_session* s, const DensifyPlan& pl, const c3d_schedule
             u.up_inv2s = 1.0f / (2.0f * sch->jbu_sigma_spatial * sch->jbu_sigma_spatial);
             u.up_inv2c = 1.0f / (2.0f * sch->jbu_sigma_color * sch->jbu_sigma_color);
             u.ntiles = pix_tiles(items, (int)pl.n);
+            HIP_TRY(launch_pix_tile_prefix(u.batch, u.nitems, s->tile_pre.p, st));
+            u.tile_pre = s->tile_pre.p;
             HIP_TRY(launch_upsample(u, st));
             if (use_prior)
                 HIP_TRY(hipMemcpyAsync(s->prior.p, s->plane.p, (size_t)npix * sizeof(float4), hipMemcpyDeviceToDevice, st));
@@ -1063,6 +1068,7 @@ int enqueue_densify(c3d_pm_session* s, const DensifyPlan& pl, const c3d_schedule
             // detail restorer (densify.rs:373-409): c_init, photometric run, c_photo, merge
             KParams e = plan_params(s, pl, level, false);
             e.ntiles = pix_tiles(items, (int)pl.n);
+            e.tile_pre = s->tile_pre.p;  // the 64 x 4 table built for the upsampling
             HIP_TRY(s->ks->eval(e, 0, st));
             HIP_TRY(hipMemcpyAsync(s->cinit.p, s->cost.p, (size_t)npix * sizeof(float), hipMemcpyDeviceToDevice, st));
             KParams k = plan_params(s, pl, level, false);
@@ -1071,6 +1077,7 @@ int enqueue_densify(c3d_pm_session* s, const DensifyPlan& pl, const c3d_schedule
             k.use_prior = use_prior ? 1 : 0;
             int rc = launch_run(s, k, iv, s->pm_copy, false, sch->photometric_iters_upper, timed);
             if (rc) return rc;
+            HIP_TRY(launch_pix_tile_prefix(e.batch, e.nitems, s->tile_pre.p, st));  // the run rebuilt it
             HIP_TRY(s->ks->eval(e, 0, st));
             HIP_TRY(launch_restore(s->plane.p, s->up.p, s->cinit.p, s->cost.p, sch->restorer_threshold, npix, st));
         }
@@ -1093,6 +1100,8 @@ int enqueue_densify(c3d_pm_session* s, const DensifyPlan& pl, const c3d_schedule
         HIP_TRY(hipMemcpyAsync(s->prior.p, s->plane.p, (size_t)npix * sizeof(float4), hipMemcpyDeviceToDevice, st));
         KParams m = plan_params(s, pl, pl.top, false);
         m.ntiles = pix_tiles(items, (int)pl.n);
+        HIP_TRY(launch_pix_tile_prefix(m.batch, m.nitems, s->tile_pre.p, st));
+        m.tile_pre = s->tile_pre.p;
         HIP_TRY(launch_median(m, st));
     }
     HIP_TRY(launch_extract_depth(s->plane.p, s->snap.p, npix, st));
