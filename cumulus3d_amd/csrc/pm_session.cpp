@@ -21,6 +21,7 @@
 #include <cstring>
 #include <mutex>
 #include <string>
+#include <thread>
 #include <vector>
 
 #include "../../include/c3d_init.h"
@@ -120,6 +121,29 @@ struct DevBuf {
     }
 };
 
+// Page-locked host memory (hipHostMalloc) that grows on demand and is freed with its owner.
+struct PinnedBuf {
+    unsigned char* p = nullptr;
+    size_t n = 0;
+    PinnedBuf() = default;
+    PinnedBuf(const PinnedBuf&) = delete;
+    PinnedBuf& operator=(const PinnedBuf&) = delete;
+    ~PinnedBuf() { release(); }
+    hipError_t reserve(size_t want) {
+        if (want <= n && p) return hipSuccess;
+        release();
+        hipError_t e = hipHostMalloc((void**)&p, std::max<size_t>(want, 1), hipHostMallocDefault);
+        if (e == hipSuccess) n = std::max<size_t>(want, 1);
+        else p = nullptr;
+        return e;
+    }
+    void release() {
+        if (p) (void)hipHostFree(p);
+        p = nullptr;
+        n = 0;
+    }
+};
+
 }  // namespace
 
 namespace {
@@ -182,8 +206,10 @@ struct c3d_pm_session {
     DevBuf<float4> low, up;
     DevBuf<float> cinit;
     DevBuf<uint8_t> bview, count, need;
-    // downloads: the final planes split into depth and 3-float normal arrays
+    // downloads: the final planes split into depth and 3-float normal arrays, and two page-locked
+    // staging buffers that the device fills while host threads empty the other one
     DevBuf<float> dl_depth, dl_normal;
+    PinnedBuf dl_stage[2];
     // first half-step tile of each batch item (launch_half_tile_prefix), rebuilt by every run
     DevBuf<int> tile_pre;
     // probes
@@ -750,6 +776,8 @@ void c3d_pm_end(c3d_pm_session* s) {
     s->d_snap_lv.release();
     s->dl_depth.release();
     s->dl_normal.release();
+    s->dl_stage[0].release();
+    s->dl_stage[1].release();
     s->tile_pre.release();
     c3d_pm_backend* be = s->be;
     delete s;  // unlocks the backend's session mutex
@@ -1482,6 +1510,54 @@ int c3d_pm_densify_learned(c3d_pm_session* s, c3d_init_net* net, const uint32_t*
 }  // extern "C"
 
 namespace {
+// One result array's device source and host destination.
+struct Download {
+    const unsigned char* src;
+    unsigned char* dst;
+    size_t bytes;
+};
+
+// A piece of a result array in a staging buffer: its offset there, its destination, its size.
+struct Staged {
+    size_t at;
+    unsigned char* dst;
+    size_t bytes;
+};
+
+// Copies staged pieces to their destinations on up to 16 host threads, each taking an equal share
+// of the bytes.
+void copy_out(const unsigned char* stage, const std::vector<Staged>& pieces) {
+    size_t total = 0;
+    for (const Staged& d : pieces) total += d.bytes;
+    if (total == 0) return;
+    const unsigned hw = std::max(1u, std::min(16u, std::thread::hardware_concurrency()));
+    const unsigned nt = (unsigned)std::min<size_t>(hw, (total + (4u << 20) - 1) / (4u << 20));
+    auto work = [&](size_t lo, size_t hi) {  // bytes [lo, hi) of the concatenated pieces
+        size_t at = 0;
+        for (const Staged& d : pieces) {
+            const size_t a = std::max(lo, at), b = std::min(hi, at + d.bytes);
+            if (a < b) std::memcpy(d.dst + (a - at), stage + d.at + (a - at), b - a);
+            at += d.bytes;
+            if (at >= hi) break;
+        }
+    };
+    if (nt <= 1) {
+        work(0, total);
+        return;
+    }
+    std::vector<std::thread> th;
+    th.reserve(nt - 1);
+    const size_t share = (total + nt - 1) / nt;
+    for (unsigned t = 1; t < nt; ++t) th.emplace_back(work, std::min(total, t * share), std::min(total, (t + 1) * share));
+    work(0, std::min(total, share));
+    for (std::thread& x : th) x.join();
+}
+
+// The top-scale results of a densify call copied into the caller's arrays: the planes are split
+// into depth and 3-float normal arrays on the device; then every array goes through two
+// page-locked staging buffers of DL_STAGE bytes: while the device copies into one, host threads
+// copy the other out to the caller's (pageable) arrays.
+constexpr size_t DL_STAGE = 256u << 20;
 int download_results(c3d_pm_session* s, const BatchItem* items, uint32_t n, long long npix, c3d_view_state* out,
                      uint8_t* const* counts, float* const* raw_depth) {
     hipStream_t st = s->be->stream;
@@ -1490,19 +1566,62 @@ int download_results(c3d_pm_session* s, const BatchItem* items, uint32_t n, long
         HIP_TRY(s->dl_normal.reserve(3 * (size_t)npix));
         HIP_TRY(launch_split_planes(s->plane2.p, s->dl_depth.p, s->dl_normal.p, npix, st));
     }
+    std::vector<Download> xs;
     for (uint32_t i = 0; i < n; ++i) {
         const size_t np = (size_t)items[i].w * items[i].h;
         const long long off = items[i].off;
+        auto add = [&](const void* src, void* dst, size_t bytes) {
+            if (dst && bytes) xs.push_back({(const unsigned char*)src, (unsigned char*)dst, bytes});
+        };
         if (out) {
-            HIP_TRY(hipMemcpyAsync(out[i].depth, s->dl_depth.p + off, np * sizeof(float), hipMemcpyDeviceToHost, st));
-            HIP_TRY(hipMemcpyAsync(out[i].normal, s->dl_normal.p + 3 * off, 3 * np * sizeof(float), hipMemcpyDeviceToHost, st));
-            if (out[i].cost) HIP_TRY(hipMemcpyAsync(out[i].cost, s->cost.p + off, np * sizeof(float), hipMemcpyDeviceToHost, st));
+            add(s->dl_depth.p + off, out[i].depth, np * sizeof(float));
+            add(s->dl_normal.p + 3 * off, out[i].normal, 3 * np * sizeof(float));
+            add(s->cost.p + off, out[i].cost, np * sizeof(float));
         }
-        if (raw_depth && raw_depth[i])
-            HIP_TRY(hipMemcpyAsync(raw_depth[i], s->snap.p + off, np * sizeof(float), hipMemcpyDeviceToHost, st));
-        if (counts && counts[i]) HIP_TRY(hipMemcpyAsync(counts[i], s->count.p + off, np, hipMemcpyDeviceToHost, st));
+        if (raw_depth) add(s->snap.p + off, raw_depth[i], np * sizeof(float));
+        if (counts) add(s->count.p + off, counts[i], np);
     }
-    HIP_TRY(hipStreamSynchronize(st));
+    if (xs.empty()) {
+        HIP_TRY(hipStreamSynchronize(st));
+        return C3D_OK;
+    }
+    HIP_TRY(s->dl_stage[0].reserve(DL_STAGE));
+    HIP_TRY(s->dl_stage[1].reserve(DL_STAGE));
+    std::vector<Staged> staged[2];
+    bool busy[2] = {false, false};
+    int cur = 0;
+    size_t fill = 0;
+    auto drain = [&](int b) -> hipError_t {
+        if (!busy[b]) return hipSuccess;
+        hipError_t e = hipEventSynchronize(s->ev[6 + b]);
+        if (e != hipSuccess) return e;
+        copy_out(s->dl_stage[b].p, staged[b]);
+        staged[b].clear();
+        busy[b] = false;
+        return hipSuccess;
+    };
+    auto close_buffer = [&]() -> hipError_t {  // the device copies into `cur` are queued: switch
+        hipError_t e = hipEventRecord(s->ev[6 + cur], st);
+        if (e != hipSuccess) return e;
+        busy[cur] = true;
+        cur = 1 - cur;
+        fill = 0;
+        return drain(cur);  // the other buffer's pieces out while the device fills this one
+    };
+    for (const Download& x : xs) {
+        size_t done = 0;
+        while (done < x.bytes) {
+            const size_t take = std::min(x.bytes - done, DL_STAGE - fill);
+            HIP_TRY(hipMemcpyAsync(s->dl_stage[cur].p + fill, x.src + done, take, hipMemcpyDeviceToHost, st));
+            staged[cur].push_back({fill, x.dst + done, take});
+            fill += take;
+            done += take;
+            if (fill == DL_STAGE) HIP_TRY(close_buffer());
+        }
+    }
+    if (fill > 0) HIP_TRY(close_buffer());
+    HIP_TRY(drain(0));
+    HIP_TRY(drain(1));
     return C3D_OK;
 }
 }  // namespace

@@ -202,10 +202,24 @@ class InitNet:
         return depth, normal, conf, head
 
 
+def output_buffers(inp, views: Sequence[int]):
+    """(states, counts, raw depth) arrays for the top-scale results of `views`, for densify_learned's
+    `out` (a caller that keeps them between jobs)."""
+    top = inp.num_levels - 1
+    outs, counts, raws = [], [], []
+    for v in views:
+        h, w = inp.level_shape(v, top)
+        outs.append(abi.ViewState(w, h))
+        counts.append(np.zeros((h, w), np.uint8))
+        raws.append(np.zeros((h, w), np.float32))
+    return outs, counts, raws
+
+
 def densify_learned(session: pm.Session, net: InitNet, views: Sequence[int], schedule: LearnedSchedule, download: bool = True,
-                    want_init: bool = False):
+                    want_init: bool = False, out=None):
     """c3d_pm_densify_learned: returns (states, counts, raw depth, timing, init depth, init normal)
-    at the top scale; the arrays are None without download / want_init."""
+    at the top scale; the arrays are None without download / want_init. `out`: arrays from
+    output_buffers to download into instead of new ones."""
     L = net.L
     inp = session.inp
     top = inp.num_levels - 1
@@ -213,12 +227,9 @@ def densify_learned(session: pm.Session, net: InitNet, views: Sequence[int], sch
     tm = abi.c3d_timing()
     outs = counts = raws = idep = inor = None
     if download:
-        outs, counts, raws = [], [], []
-        for v in views:
-            h, w = inp.level_shape(v, top)
-            outs.append(abi.ViewState(w, h))
-            counts.append(np.zeros((h, w), np.uint8))
-            raws.append(np.zeros((h, w), np.float32))
+        outs, counts, raws = out if out is not None else output_buffers(inp, views)
+        if len(outs) != len(views) or len(counts) != len(views) or len(raws) != len(views):
+            raise ValueError("out holds arrays for a different number of views")
     if want_init:
         idep, inor = [], []
         for v in views:
