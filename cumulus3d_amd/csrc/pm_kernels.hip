@@ -945,6 +945,18 @@ hipError_t launch_sample_reciprocal(const float* z, float* r, long long n, hipSt
     return hipGetLastError();
 }
 
+// a depth array and a 3-float normal array merged into planes (nx, ny, nz, d) (state uploads)
+__global__ __launch_bounds__(256) void k_merge_planes(const float* depth, const float* normal, float4* plane, long long n) {
+    const long long i = (long long)blockIdx.x * 256 + threadIdx.x;
+    if (i < n) plane[i] = make_float4(normal[3 * i], normal[3 * i + 1], normal[3 * i + 2], depth[i]);
+}
+
+hipError_t launch_merge_planes(const float* depth, const float* normal, float4* plane, long long n, hipStream_t s) {
+    if (n <= 0) return hipSuccess;
+    hipLaunchKernelGGL(k_merge_planes, dim3((unsigned)((n + 255) / 256)), dim3(256), 0, s, depth, normal, plane, n);
+    return hipGetLastError();
+}
+
 // planes (nx, ny, nz, d) split into a depth array and a 3-float normal array (result downloads)
 __global__ __launch_bounds__(256) void k_split_planes(const float4* plane, float* depth, float* normal, long long n) {
     const long long i = (long long)blockIdx.x * 256 + threadIdx.x;

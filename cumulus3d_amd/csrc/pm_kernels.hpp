@@ -34,6 +34,7 @@ hipError_t launch_sample_reciprocal(const float* z, float* r, long long n, hipSt
 // random initialisation by P.random_init and best-view reset of every pixel (P.ntiles = pix_tiles)
 hipError_t launch_eval_init(const KParams& P, long long npix, hipStream_t s);
 hipError_t launch_split_planes(const float4* plane, float* depth, float* normal, long long n, hipStream_t s);
+hipError_t launch_merge_planes(const float* depth, const float* normal, float4* plane, long long n, hipStream_t s);
 hipError_t launch_upsample(const KParams& P, hipStream_t s);
 hipError_t launch_median(const KParams& P, hipStream_t s);
 hipError_t launch_quad_build(const uint8_t* g, int w, int h, uint32_t* q, hipStream_t s);

@@ -220,7 +220,6 @@ struct c3d_pm_session {
     c3d_pm_params pm_copy{};
     c3d_filter_params filter_copy{};
     std::vector<float4> host4;
-    std::vector<float> host1;
     hipEvent_t ev[8] = {};
     // per-launch timing of the half-step kernels: pairs of events, read after the final sync
     std::vector<hipEvent_t> pool;
@@ -341,41 +340,16 @@ KParams params_for(c3d_pm_session* s, uint32_t level, bool geometric, const std:
     return k;
 }
 
-// Pack host states (SoA) into float4 planes at the item offsets (patchmatch.rs:327-336).
-void pack(c3d_pm_session* s, const c3d_view_state* st, const std::vector<BatchItem>& items, long long npix, bool prior) {
-    s->host4.resize((size_t)npix);
-    for (size_t i = 0; i < items.size(); ++i) {
-        const c3d_view_state& v = st[i];
-        float4* o = s->host4.data() + items[i].off;
-        size_t n = (size_t)v.width * v.height;
-        if (prior) {
-            std::memcpy(o, v.prior, n * sizeof(float4));
-        } else {
-            for (size_t p = 0; p < n; ++p) o[p] = make_float4(v.normal[3 * p], v.normal[3 * p + 1], v.normal[3 * p + 2], v.depth[p]);
-        }
-    }
-}
-
-void unpack(const std::vector<float4>& h, const std::vector<BatchItem>& items, c3d_view_state* st) {
-    for (size_t i = 0; i < items.size(); ++i) {
-        c3d_view_state& v = st[i];
-        const float4* p = h.data() + items[i].off;
-        size_t n = (size_t)v.width * v.height;
-        for (size_t q = 0; q < n; ++q) {
-            v.normal[3 * q] = p[q].x;
-            v.normal[3 * q + 1] = p[q].y;
-            v.normal[3 * q + 2] = p[q].z;
-            v.depth[q] = p[q].w;
-        }
-    }
-}
-
 // The top-scale results of a densify call copied straight into the caller's arrays: the planes
 // are split into depth and 3-float normal arrays on the device, then each view's depth, normal,
 // cost, raw depth and count come back with one transfer each, in stream order, and one final
 // synchronisation (no host staging buffer, no host-side unpacking).
 int download_results(c3d_pm_session* s, const BatchItem* items, uint32_t n, long long npix, c3d_view_state* out,
                      uint8_t* const* counts, float* const* raw_depth);
+int download_states(c3d_pm_session* s, const float4* plane, const float* cost, const BatchItem* items, uint32_t n, long long npix,
+                    c3d_view_state* out, uint8_t* const* counts, float* const* raw_depth);
+int upload_states(c3d_pm_session* s, const c3d_view_state* st, const std::vector<BatchItem>& items, long long npix, bool prior,
+                  float4* dst);
 
 // Snapshot upload, only when the id changes (patchmatch.rs:573-604).
 // The snapshot's maps are laid out with the sizes of snap->level and read with the sizes of the
@@ -806,15 +780,13 @@ int c3d_pm_run(c3d_pm_session* s, const uint32_t* views, uint32_t n, c3d_view_st
         for (uint32_t i = a; i < b; ++i)
             if (!states[i].prior) use_prior = false;
         if (p->random_init != 1) {
-            pack(s, states + a, items, npix, false);
-            HIP_TRY(hipMemcpyAsync(s->plane.p, s->host4.data(), (size_t)npix * sizeof(float4), hipMemcpyHostToDevice, st));
-            HIP_TRY(hipStreamSynchronize(st));
+            rc = upload_states(s, states + a, items, npix, false, s->plane.p);
+            if (rc) return rc;
         }
         if (use_prior) {
             HIP_TRY(s->prior.reserve(npix));
-            pack(s, states + a, items, npix, true);
-            HIP_TRY(hipMemcpyAsync(s->prior.p, s->host4.data(), (size_t)npix * sizeof(float4), hipMemcpyHostToDevice, st));
-            HIP_TRY(hipStreamSynchronize(st));
+            rc = upload_states(s, states + a, items, npix, true, s->prior.p);
+            if (rc) return rc;
         }
         KParams k = params_for(s, p->level, p->geometric != 0, items, max_nsrc);
         k.run_id = (int)p->run_id;
@@ -822,15 +794,8 @@ int c3d_pm_run(c3d_pm_session* s, const uint32_t* views, uint32_t n, c3d_view_st
         k.use_prior = use_prior ? 1 : 0;
         rc = launch_run(s, k, items, s->pm_copy, p->geometric != 0, p->iterations, false);
         if (rc) return rc;
-        s->host4.resize((size_t)npix);
-        s->host1.resize((size_t)npix);
-        HIP_TRY(hipMemcpyAsync(s->host4.data(), s->plane.p, (size_t)npix * sizeof(float4), hipMemcpyDeviceToHost, st));
-        HIP_TRY(hipMemcpyAsync(s->host1.data(), s->cost.p, (size_t)npix * sizeof(float), hipMemcpyDeviceToHost, st));
-        HIP_TRY(hipStreamSynchronize(st));
-        unpack(s->host4, items, states + a);
-        for (size_t i = 0; i < items.size(); ++i)
-            if (states[a + i].cost)
-                std::memcpy(states[a + i].cost, s->host1.data() + items[i].off, (size_t)items[i].w * items[i].h * sizeof(float));
+        rc = download_states(s, s->plane.p, s->cost.p, items.data(), (uint32_t)items.size(), npix, states + a, nullptr, nullptr);
+        if (rc) return rc;
     }
     return C3D_OK;
 }
@@ -853,16 +818,15 @@ int c3d_pm_evaluate(c3d_pm_session* s, uint32_t level, const uint32_t* views, ui
         int max_nsrc;
         rc = set_batch(s, views, a, b, level, items, npix, max_nsrc);
         if (rc) return rc;
-        pack(s, states + a, items, npix, false);
-        HIP_TRY(hipMemcpyAsync(s->plane.p, s->host4.data(), (size_t)npix * sizeof(float4), hipMemcpyHostToDevice, st));
+        rc = upload_states(s, states + a, items, npix, false, s->plane.p);
+        if (rc) return rc;
         KParams k = params_for(s, level, geometric != 0, items, max_nsrc);
         k.ntiles = pix_tiles(items.data(), (int)items.size());
         HIP_TRY(s->ks->eval(k, 0, st));
-        s->host1.resize((size_t)npix);
-        HIP_TRY(hipMemcpyAsync(s->host1.data(), s->cost.p, (size_t)npix * sizeof(float), hipMemcpyDeviceToHost, st));
-        HIP_TRY(hipStreamSynchronize(st));
         for (size_t i = 0; i < items.size(); ++i)
-            std::memcpy(costs[a + i], s->host1.data() + items[i].off, (size_t)items[i].w * items[i].h * sizeof(float));
+            HIP_TRY(hipMemcpyAsync(costs[a + i], s->cost.p + items[i].off, (size_t)items[i].w * items[i].h * sizeof(float),
+                                   hipMemcpyDeviceToHost, st));
+        HIP_TRY(hipStreamSynchronize(st));
     }
     return C3D_OK;
 }
@@ -884,16 +848,14 @@ int c3d_pm_filter(c3d_pm_session* s, const uint32_t* views, uint32_t n, const c3
         int max_nsrc;
         rc = set_batch(s, views, a, b, level, items, npix, max_nsrc);
         if (rc) return rc;
-        pack(s, states + a, items, npix, false);
-        HIP_TRY(hipMemcpyAsync(s->plane.p, s->host4.data(), (size_t)npix * sizeof(float4), hipMemcpyHostToDevice, st));
+        rc = upload_states(s, states + a, items, npix, false, s->plane.p);
+        if (rc) return rc;
         KParams k = params_for(s, level, true, items, max_nsrc);
         k.ntiles = pix_tiles(items.data(), (int)items.size());
         HIP_TRY(s->ks->filter(k, st));
-        std::vector<uint8_t> hc((size_t)npix);
-        HIP_TRY(hipMemcpyAsync(hc.data(), s->count.p, (size_t)npix, hipMemcpyDeviceToHost, st));
-        HIP_TRY(hipStreamSynchronize(st));
         for (size_t i = 0; i < items.size(); ++i)
-            std::memcpy(counts[a + i], hc.data() + items[i].off, (size_t)items[i].w * items[i].h);
+            HIP_TRY(hipMemcpyAsync(counts[a + i], s->count.p + items[i].off, (size_t)items[i].w * items[i].h, hipMemcpyDeviceToHost, st));
+        HIP_TRY(hipStreamSynchronize(st));
     }
     return C3D_OK;
 }
@@ -929,17 +891,15 @@ int c3d_pm_upsample(c3d_pm_session* s, uint32_t level, const uint32_t* views, ui
         }
         HIP_TRY(hipMemcpyAsync(s->d_batch.p, items.data(), items.size() * sizeof(BatchItem), hipMemcpyHostToDevice, st));
         HIP_TRY(s->prior.reserve(lpix));
-        pack(s, low + a, low_items, lpix, false);
-        HIP_TRY(hipMemcpyAsync(s->prior.p, s->host4.data(), (size_t)lpix * sizeof(float4), hipMemcpyHostToDevice, st));
+        rc = upload_states(s, low + a, low_items, lpix, false, s->prior.p);
+        if (rc) return rc;
         KParams k = params_for(s, level, false, items, max_nsrc);
         k.up_inv2s = 1.0f / (2.0f * sigma_s * sigma_s);
         k.up_inv2c = 1.0f / (2.0f * sigma_c * sigma_c);
         k.ntiles = pix_tiles(items.data(), (int)items.size());
         HIP_TRY(launch_upsample(k, st));
-        s->host4.resize((size_t)npix);
-        HIP_TRY(hipMemcpyAsync(s->host4.data(), s->plane.p, (size_t)npix * sizeof(float4), hipMemcpyDeviceToHost, st));
-        HIP_TRY(hipStreamSynchronize(st));
-        unpack(s->host4, items, out + a);
+        rc = download_states(s, s->plane.p, nullptr, items.data(), (uint32_t)items.size(), npix, out + a, nullptr, nullptr);
+        if (rc) return rc;
     }
     return C3D_OK;
 }
@@ -961,15 +921,13 @@ int c3d_pm_median(c3d_pm_session* s, uint32_t level, const uint32_t* views, uint
         rc = set_batch(s, views, a, b, level, items, npix, max_nsrc);
         if (rc) return rc;
         HIP_TRY(s->prior.reserve(npix));
-        pack(s, in + a, items, npix, false);
-        HIP_TRY(hipMemcpyAsync(s->prior.p, s->host4.data(), (size_t)npix * sizeof(float4), hipMemcpyHostToDevice, st));
+        rc = upload_states(s, in + a, items, npix, false, s->prior.p);
+        if (rc) return rc;
         KParams k = params_for(s, level, false, items, max_nsrc);
         k.ntiles = pix_tiles(items.data(), (int)items.size());
         HIP_TRY(launch_median(k, st));
-        s->host4.resize((size_t)npix);
-        HIP_TRY(hipMemcpyAsync(s->host4.data(), s->plane.p, (size_t)npix * sizeof(float4), hipMemcpyDeviceToHost, st));
-        HIP_TRY(hipStreamSynchronize(st));
-        unpack(s->host4, items, out + a);
+        rc = download_states(s, s->plane.p, nullptr, items.data(), (uint32_t)items.size(), npix, out + a, nullptr, nullptr);
+        if (rc) return rc;
         for (size_t i = 0; i < items.size(); ++i)
             if (out[a + i].cost && in[a + i].cost && out[a + i].cost != in[a + i].cost)
                 std::memcpy(out[a + i].cost, in[a + i].cost, (size_t)items[i].w * items[i].h * sizeof(float));
@@ -1510,6 +1468,17 @@ int c3d_pm_densify_learned(c3d_pm_session* s, c3d_init_net* net, const uint32_t*
 }  // extern "C"
 
 namespace {
+// The depth and normal temporaries of state transfers for `npix` pixels; growing them frees the old
+// ones, which queued kernels of the same call may still read, so the stream drains first.
+hipError_t grow_temp(c3d_pm_session* s, long long npix) {
+    if ((size_t)npix <= s->dl_depth.n && 3 * (size_t)npix <= s->dl_normal.n && s->dl_depth.p && s->dl_normal.p) return hipSuccess;
+    hipError_t e = hipStreamSynchronize(s->be->stream);
+    if (e != hipSuccess) return e;
+    e = s->dl_depth.reserve((size_t)npix);
+    if (e != hipSuccess) return e;
+    return s->dl_normal.reserve(3 * (size_t)npix);
+}
+
 // One result array's device source and host destination.
 struct Download {
     const unsigned char* src;
@@ -1558,13 +1527,12 @@ void copy_out(const unsigned char* stage, const std::vector<Staged>& pieces) {
 // page-locked staging buffers of DL_STAGE bytes: while the device copies into one, host threads
 // copy the other out to the caller's (pageable) arrays.
 constexpr size_t DL_STAGE = 256u << 20;
-int download_results(c3d_pm_session* s, const BatchItem* items, uint32_t n, long long npix, c3d_view_state* out,
-                     uint8_t* const* counts, float* const* raw_depth) {
+int download_states(c3d_pm_session* s, const float4* plane, const float* cost, const BatchItem* items, uint32_t n, long long npix,
+                    c3d_view_state* out, uint8_t* const* counts, float* const* raw_depth) {
     hipStream_t st = s->be->stream;
     if (out) {
-        HIP_TRY(s->dl_depth.reserve((size_t)npix));
-        HIP_TRY(s->dl_normal.reserve(3 * (size_t)npix));
-        HIP_TRY(launch_split_planes(s->plane2.p, s->dl_depth.p, s->dl_normal.p, npix, st));
+        HIP_TRY(grow_temp(s, npix));
+        HIP_TRY(launch_split_planes(plane, s->dl_depth.p, s->dl_normal.p, npix, st));
     }
     std::vector<Download> xs;
     for (uint32_t i = 0; i < n; ++i) {
@@ -1576,7 +1544,7 @@ int download_results(c3d_pm_session* s, const BatchItem* items, uint32_t n, long
         if (out) {
             add(s->dl_depth.p + off, out[i].depth, np * sizeof(float));
             add(s->dl_normal.p + 3 * off, out[i].normal, 3 * np * sizeof(float));
-            add(s->cost.p + off, out[i].cost, np * sizeof(float));
+            if (cost) add(cost + off, out[i].cost, np * sizeof(float));
         }
         if (raw_depth) add(s->snap.p + off, raw_depth[i], np * sizeof(float));
         if (counts) add(s->count.p + off, counts[i], np);
@@ -1622,6 +1590,33 @@ int download_results(c3d_pm_session* s, const BatchItem* items, uint32_t n, long
     if (fill > 0) HIP_TRY(close_buffer());
     HIP_TRY(drain(0));
     HIP_TRY(drain(1));
+    return C3D_OK;
+}
+
+int download_results(c3d_pm_session* s, const BatchItem* items, uint32_t n, long long npix, c3d_view_state* out,
+                     uint8_t* const* counts, float* const* raw_depth) {
+    return download_states(s, s->plane2.p, s->cost.p, items, n, npix, out, counts, raw_depth);
+}
+
+// Caller states into device planes at the item offsets (patchmatch.rs:327-336): each view's depth
+// and normals (or, with `prior`, its float4 prior planes) go to the device with one transfer each,
+// and the planes are assembled there.
+int upload_states(c3d_pm_session* s, const c3d_view_state* st, const std::vector<BatchItem>& items, long long npix, bool prior,
+                  float4* dst) {
+    hipStream_t sm = s->be->stream;
+    if (prior) {
+        for (size_t i = 0; i < items.size(); ++i)
+            HIP_TRY(hipMemcpyAsync(dst + items[i].off, st[i].prior, (size_t)items[i].w * items[i].h * sizeof(float4),
+                                   hipMemcpyHostToDevice, sm));
+        return C3D_OK;
+    }
+    HIP_TRY(grow_temp(s, npix));
+    for (size_t i = 0; i < items.size(); ++i) {
+        const size_t np = (size_t)items[i].w * items[i].h;
+        HIP_TRY(hipMemcpyAsync(s->dl_depth.p + items[i].off, st[i].depth, np * sizeof(float), hipMemcpyHostToDevice, sm));
+        HIP_TRY(hipMemcpyAsync(s->dl_normal.p + 3 * items[i].off, st[i].normal, 3 * np * sizeof(float), hipMemcpyHostToDevice, sm));
+    }
+    HIP_TRY(launch_merge_planes(s->dl_depth.p, s->dl_normal.p, dst, npix, sm));
     return C3D_OK;
 }
 }  // namespace
