@@ -9,6 +9,31 @@
 
 #include "../../include/c3d_pm.h"
 
+namespace c3d {
+// Page-locked host memory (hipHostMalloc) that grows on demand and is freed with its owner.
+struct PinnedBuf {
+    unsigned char* p = nullptr;
+    size_t n = 0;
+    PinnedBuf() = default;
+    PinnedBuf(const PinnedBuf&) = delete;
+    PinnedBuf& operator=(const PinnedBuf&) = delete;
+    ~PinnedBuf() { release(); }
+    hipError_t reserve(size_t want) {
+        if (want <= n && p) return hipSuccess;
+        release();
+        hipError_t e = hipHostMalloc((void**)&p, want > 0 ? want : 1, hipHostMallocDefault);
+        if (e == hipSuccess) n = want > 0 ? want : 1;
+        else p = nullptr;
+        return e;
+    }
+    void release() {
+        if (p) (void)hipHostFree(p);
+        p = nullptr;
+        n = 0;
+    }
+};
+}  // namespace c3d
+
 struct c3d_pm_backend {
     int device = 0;
     std::string name;
@@ -20,6 +45,9 @@ struct c3d_pm_backend {
     std::mutex life;
     int holders = 0;
     bool destroy_pending = false;
+    // the two page-locked staging buffers of result downloads, used by the session that holds
+    // `lock` and kept for the next one (page-locking hundreds of MB is not free)
+    c3d::PinnedBuf dl_stage[2];
 };
 
 namespace c3d {

@@ -121,28 +121,6 @@ struct DevBuf {
     }
 };
 
-// Page-locked host memory (hipHostMalloc) that grows on demand and is freed with its owner.
-struct PinnedBuf {
-    unsigned char* p = nullptr;
-    size_t n = 0;
-    PinnedBuf() = default;
-    PinnedBuf(const PinnedBuf&) = delete;
-    PinnedBuf& operator=(const PinnedBuf&) = delete;
-    ~PinnedBuf() { release(); }
-    hipError_t reserve(size_t want) {
-        if (want <= n && p) return hipSuccess;
-        release();
-        hipError_t e = hipHostMalloc((void**)&p, std::max<size_t>(want, 1), hipHostMallocDefault);
-        if (e == hipSuccess) n = std::max<size_t>(want, 1);
-        else p = nullptr;
-        return e;
-    }
-    void release() {
-        if (p) (void)hipHostFree(p);
-        p = nullptr;
-        n = 0;
-    }
-};
 
 }  // namespace
 
@@ -206,10 +184,9 @@ struct c3d_pm_session {
     DevBuf<float4> low, up;
     DevBuf<float> cinit;
     DevBuf<uint8_t> bview, count, need;
-    // downloads: the final planes split into depth and 3-float normal arrays, and two page-locked
-    // staging buffers that the device fills while host threads empty the other one
+    // downloads: the final planes split into depth and 3-float normal arrays (the page-locked
+    // staging buffers belong to the backend)
     DevBuf<float> dl_depth, dl_normal;
-    PinnedBuf dl_stage[2];
     // first half-step tile of each batch item (launch_half_tile_prefix), rebuilt by every run
     DevBuf<int> tile_pre;
     // probes
@@ -750,8 +727,6 @@ void c3d_pm_end(c3d_pm_session* s) {
     s->d_snap_lv.release();
     s->dl_depth.release();
     s->dl_normal.release();
-    s->dl_stage[0].release();
-    s->dl_stage[1].release();
     s->tile_pre.release();
     c3d_pm_backend* be = s->be;
     delete s;  // unlocks the backend's session mutex
@@ -1553,8 +1528,9 @@ int download_states(c3d_pm_session* s, const float4* plane, const float* cost, c
         HIP_TRY(hipStreamSynchronize(st));
         return C3D_OK;
     }
-    HIP_TRY(s->dl_stage[0].reserve(DL_STAGE));
-    HIP_TRY(s->dl_stage[1].reserve(DL_STAGE));
+    PinnedBuf* stage = s->be->dl_stage;
+    HIP_TRY(stage[0].reserve(DL_STAGE));
+    HIP_TRY(stage[1].reserve(DL_STAGE));
     std::vector<Staged> staged[2];
     bool busy[2] = {false, false};
     int cur = 0;
@@ -1563,7 +1539,7 @@ int download_states(c3d_pm_session* s, const float4* plane, const float* cost, c
         if (!busy[b]) return hipSuccess;
         hipError_t e = hipEventSynchronize(s->ev[6 + b]);
         if (e != hipSuccess) return e;
-        copy_out(s->dl_stage[b].p, staged[b]);
+        copy_out(stage[b].p, staged[b]);
         staged[b].clear();
         busy[b] = false;
         return hipSuccess;
@@ -1580,7 +1556,7 @@ int download_states(c3d_pm_session* s, const float4* plane, const float* cost, c
         size_t done = 0;
         while (done < x.bytes) {
             const size_t take = std::min(x.bytes - done, DL_STAGE - fill);
-            HIP_TRY(hipMemcpyAsync(s->dl_stage[cur].p + fill, x.src + done, take, hipMemcpyDeviceToHost, st));
+            HIP_TRY(hipMemcpyAsync(stage[cur].p + fill, x.src + done, take, hipMemcpyDeviceToHost, st));
             staged[cur].push_back({fill, x.dst + done, take});
             fill += take;
             done += take;
