@@ -389,12 +389,12 @@ __device__ __forceinline__ float quad_sample(const uint32_t* qbase, float qp4, f
     return quad_lerp(q, ax, ay);
 }
 
-// Gathers kept in flight by the window loop: this default serves the evaluation, read-out and
-// cost-probe kernels (8: k_eval 5.93 -> 5.85 ms, k_filter 4.39 -> 4.35 ms against 6 on
-// 7 x 1920x1080, 6 sources; both stay at two waves per SIMD); the half step picks its own per
-// build. There, on the same job: 4 -> 502.8 ms, 6 -> 500.8 ms, 8 -> 500.8 ms (255 VGPRs:
-// photometric half steps 2.9 ms faster, geometric ones 2.8 ms slower), 12 spills; the compiler's
-// own batches of eight gathers drained to zero before the next batch gave 512.9 ms.
+// Gathers kept in flight by the window loop: this default serves the cost-probe kernel; the
+// evaluation and read-out keep five (three waves per SIMD) and the half step picks its own per
+// build. Measured on 7 x 1920x1080, 6 sources, with 4-row half-step workgroups: 4 -> 502.8 ms,
+// 6 -> 500.8 ms, 8 -> 500.8 ms per job (photometric half steps 2.9 ms faster, geometric ones
+// 2.8 ms slower), 12 spills; the compiler's own batches of eight gathers drained to zero before
+// the next batch gave 512.9 ms.
 constexpr int WINDOW_PIPE = 8;
 
 // Window sums of one hypothesis x source (H translated to the pixel). A sample whose projective

@@ -23,7 +23,7 @@ struct KernelSet {
 // outside those bounds.
 const KernelSet* kernel_set(int wr, int wstep, bool any_radius);
 
-// half-step workgroup shapes compiled: (rows of a workgroup, lanes per pixel) = (2, 1), (4, 1), (4, 2)
+// half-step workgroup shapes compiled: (rows of a workgroup, lanes per pixel) = (1, 1), (2, 1), (4, 1), (4, 2)
 bool half_shape_ok(int rows, int tpp);
 
 // workgroup counts of the two tilings for a batch
