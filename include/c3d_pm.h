@@ -192,7 +192,9 @@ const char* c3d_pm_name(const c3d_pm_backend* backend);
  * its lifetime in the reference (kernel.rs:244). Returns 0 when the backend was freed now; otherwise
  * the number of sessions and networks still open: the backend is then freed when the last of them
  * is closed (c3d_pm_end, c3d_init_destroy), and every further c3d_pm_begin or c3d_init_create on
- * it fails with C3D_INVALID_ARGUMENT. The handle must not be used by the caller after this call. */
+ * it fails with C3D_INVALID_ARGUMENT. The handle must not be used by the caller after this call.
+ * A backend keeps two 256 MB page-locked host buffers, through which sessions download their
+ * results, from the first download until it is freed. */
 int c3d_pm_destroy(c3d_pm_backend* backend);
 
 /* 1 when a gfx950 device can be opened; is_available (crates/cuda/src/device.rs:80-89). */
