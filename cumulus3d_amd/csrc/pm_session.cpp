@@ -18,6 +18,7 @@
 #include <chrono>
 #include <cmath>
 #include <cstdio>
+#include <cstdlib>
 #include <cstring>
 #include <mutex>
 #include <string>
@@ -1528,9 +1529,16 @@ int download_states(c3d_pm_session* s, const float4* plane, const float* cost, c
         HIP_TRY(hipStreamSynchronize(st));
         return C3D_OK;
     }
+    // $C3D_PM_STAGE_BYTES (64 bytes or more) shrinks the staging buffers, so that tests can switch
+    // between them many times on small jobs
+    size_t stage_bytes = DL_STAGE;
+    if (const char* e = std::getenv("C3D_PM_STAGE_BYTES")) {
+        const long long v = std::atoll(e);
+        if (v >= 64 && (size_t)v < DL_STAGE) stage_bytes = (size_t)v;
+    }
     PinnedBuf* stage = s->be->dl_stage;
-    HIP_TRY(stage[0].reserve(DL_STAGE));
-    HIP_TRY(stage[1].reserve(DL_STAGE));
+    HIP_TRY(stage[0].reserve(stage_bytes));
+    HIP_TRY(stage[1].reserve(stage_bytes));
     std::vector<Staged> staged[2];
     bool busy[2] = {false, false};
     int cur = 0;
@@ -1555,12 +1563,12 @@ int download_states(c3d_pm_session* s, const float4* plane, const float* cost, c
     for (const Download& x : xs) {
         size_t done = 0;
         while (done < x.bytes) {
-            const size_t take = std::min(x.bytes - done, DL_STAGE - fill);
+            const size_t take = std::min(x.bytes - done, stage_bytes - fill);
             HIP_TRY(hipMemcpyAsync(stage[cur].p + fill, x.src + done, take, hipMemcpyDeviceToHost, st));
             staged[cur].push_back({fill, x.dst + done, take});
             fill += take;
             done += take;
-            if (fill == DL_STAGE) HIP_TRY(close_buffer());
+            if (fill == stage_bytes) HIP_TRY(close_buffer());
         }
     }
     if (fill > 0) HIP_TRY(close_buffer());
