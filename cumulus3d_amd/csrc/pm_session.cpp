@@ -1490,10 +1490,19 @@ void copy_out(const unsigned char* stage, const std::vector<Staged>& pieces) {
         work(0, total);
         return;
     }
-    std::vector<std::thread> th;
-    th.reserve(nt - 1);
     const size_t share = (total + nt - 1) / nt;
-    for (unsigned t = 1; t < nt; ++t) th.emplace_back(work, std::min(total, t * share), std::min(total, (t + 1) * share));
+    std::vector<std::thread> th;
+    unsigned started = 1;  // shares [1, started) run on their own threads
+    try {
+        th.reserve(nt - 1);
+        for (unsigned t = 1; t < nt; ++t) {
+            th.emplace_back(work, std::min(total, t * share), std::min(total, (t + 1) * share));
+            started = t + 1;
+        }
+    } catch (...) {
+        // no thread (or no memory for one): the calling thread takes the shares that did not start
+    }
+    for (unsigned t = started; t < nt; ++t) work(std::min(total, t * share), std::min(total, (t + 1) * share));
     work(0, std::min(total, share));
     for (std::thread& x : th) x.join();
 }
