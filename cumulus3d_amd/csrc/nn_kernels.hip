@@ -3,8 +3,9 @@
 // the training loss against plain-PatchStereo labels and the plane-sweep input features.
 //
 // Convolutions are implicit GEMMs over NHWC tensors: the 64 lanes of a wave load 16 pixels x 32
-// channels (one 16-byte load per lane) as the A operand and 32 channels x 16 output channels of
-// the weights as the B operand; a 256-thread workgroup computes 64 pixels x 64 output channels.
+// channels (one 16-byte load per lane) as an A operand and 32 channels x 16 output channels of
+// the weights as the B operand, which serves the wave's 1 or 4 groups of 16 pixels; a 256-thread
+// workgroup computes 64 or 256 pixels x 64 output channels.
 // The weight gradient stages 32 pixels x 32 channels of both operands transposed in LDS so that
 // the reduction over pixels is the MFMA's K dimension.
 #include "nn_kernels.hpp"
@@ -29,71 +30,100 @@ __device__ __forceinline__ f32x4 mfma(s16x8 a, s16x8 b, f32x4 c) { return __buil
 
 // ---------------------------------------------------------------- convolution (forward / data gradient)
 
+// G pixel groups of 16 per wave: the wave loads each B operand (32 input x 16 output channels of
+// one tap) once for its G groups. Every output still accumulates tap by tap and 32 channels at a
+// time in the same order, so G changes no result.
+template <int G>
 __global__ __launch_bounds__(256) void k_conv(ConvArgs a) {
     const int lane = threadIdx.x & 63, wave = threadIdx.x >> 6;
     const int r16 = lane & 15, kg = lane >> 4;
     const long long HW = (long long)a.H * a.W;
     const long long P = (long long)a.nimg * HW;
-    const long long p0 = (long long)blockIdx.x * 64 + wave * 16;
+    const long long p0 = ((long long)blockIdx.x * 4 + wave) * (16 * G);
     const int co0 = blockIdx.y * 64;
     const int ntile = min(4, (a.cout - co0) / 16);
-    const long long pa = p0 + r16;
-    const bool pin = pa < P;
-    int img = 0, py = 0, px = 0;
-    if (pin) {
-        img = (int)(pa / HW);
-        const int rem = (int)(pa - (long long)img * HW);
-        py = rem / a.W;
-        px = rem - py * a.W;
-    }
-    f32x4 acc[4];
+    int img[G], py[G], px[G];
+    bool pin[G];
 #pragma unroll
-    for (int t = 0; t < 4; ++t) acc[t] = f32x4{0.f, 0.f, 0.f, 0.f};
+    for (int g = 0; g < G; ++g) {
+        const long long pa = p0 + 16 * g + r16;
+        pin[g] = pa < P;
+        img[g] = py[g] = px[g] = 0;
+        if (pin[g]) {
+            img[g] = (int)(pa / HW);
+            const int rem = (int)(pa - (long long)img[g] * HW);
+            py[g] = rem / a.W;
+            px[g] = rem - py[g] * a.W;
+        }
+    }
+    f32x4 acc[G][4];
+#pragma unroll
+    for (int g = 0; g < G; ++g)
+#pragma unroll
+        for (int t = 0; t < 4; ++t) acc[g][t] = f32x4{0.f, 0.f, 0.f, 0.f};
     const int taps = a.ksize * a.ksize, rad = a.ksize / 2;
     for (int t = 0; t < taps; ++t) {
-        const int qy = py + t / a.ksize - rad, qx = px + t % a.ksize - rad;
-        const bool ok = pin && qy >= 0 && qy < a.H && qx >= 0 && qx < a.W;
-        const long long xoff = ok ? (((long long)img * HW + (long long)qy * a.W + qx) * a.xs + kg * 8) : 0;
+        const int dy = t / a.ksize - rad, dx = t % a.ksize - rad;
+        bool ok[G];
+        long long xoff[G];
+#pragma unroll
+        for (int g = 0; g < G; ++g) {
+            const int qy = py[g] + dy, qx = px[g] + dx;
+            ok[g] = pin[g] && qy >= 0 && qy < a.H && qx >= 0 && qx < a.W;
+            xoff[g] = ok[g] ? (((long long)img[g] * HW + (long long)qy * a.W + qx) * a.xs + kg * 8) : 0;
+        }
         for (int c0 = 0; c0 < a.cin; c0 += 32) {
-            s16x8 A = {0, 0, 0, 0, 0, 0, 0, 0};
-            if (ok) A = *(const s16x8*)(a.x + xoff + c0);
+            s16x8 A[G];
+#pragma unroll
+            for (int g = 0; g < G; ++g) {
+                A[g] = s16x8{0, 0, 0, 0, 0, 0, 0, 0};
+                if (ok[g]) A[g] = *(const s16x8*)(a.x + xoff[g] + c0);
+            }
 #pragma unroll
             for (int tt = 0; tt < 4; ++tt) {
                 if (tt < ntile) {
                     const int co = co0 + tt * 16 + r16;
                     const s16x8 B = *(const s16x8*)(a.w + ((long long)co * taps + t) * a.cin + c0 + kg * 8);
-                    acc[tt] = mfma(A, B, acc[tt]);
+#pragma unroll
+                    for (int g = 0; g < G; ++g) acc[g][tt] = mfma(A[g], B, acc[g][tt]);
                 }
             }
         }
     }
     // lane holds rows (pixels) 4 kg + j and column (output channel) r16 of each 16 x 16 tile
 #pragma unroll
-    for (int tt = 0; tt < 4; ++tt) {
-        if (tt >= ntile) continue;
-        const int co = co0 + tt * 16 + r16;
-        const float b = a.bias ? a.bias[co] : 0.f;
+    for (int g = 0; g < G; ++g) {
 #pragma unroll
-        for (int j = 0; j < 4; ++j) {
-            const long long p = p0 + 4 * kg + j;
-            if (p >= P) continue;
-            float v = acc[tt][j] + b;
-            if (a.mode == 0) {
-                if (a.relu) v = fmaxf(v, 0.f);
-                a.y[p * a.ys + a.yoff + co] = f2bf(v);
-            } else if (a.mode == 1) {
-                a.y32[p * a.y32s + co] = v;
-            } else {
-                a.y32[p * a.y32s + co] += v;
+        for (int tt = 0; tt < 4; ++tt) {
+            if (tt >= ntile) continue;
+            const int co = co0 + tt * 16 + r16;
+            const float b = a.bias ? a.bias[co] : 0.f;
+#pragma unroll
+            for (int j = 0; j < 4; ++j) {
+                const long long p = p0 + 16 * g + 4 * kg + j;
+                if (p >= P) continue;
+                float v = acc[g][tt][j] + b;
+                if (a.mode == 0) {
+                    if (a.relu) v = fmaxf(v, 0.f);
+                    a.y[p * a.ys + a.yoff + co] = f2bf(v);
+                } else if (a.mode == 1) {
+                    a.y32[p * a.y32s + co] = v;
+                } else {
+                    a.y32[p * a.y32s + co] += v;
+                }
             }
         }
     }
 }
 
+// Four pixel groups per wave where the launch still has at least 512 workgroups (two per CU),
+// one otherwise (the small deep layers keep their parallelism).
 hipError_t conv(const ConvArgs& a, hipStream_t s) {
     const long long P = (long long)a.nimg * a.H * a.W;
-    dim3 grid((unsigned)((P + 63) / 64), (unsigned)((a.cout + 63) / 64));
-    hipLaunchKernelGGL(k_conv, grid, dim3(256), 0, s, a);
+    const unsigned ny = (unsigned)((a.cout + 63) / 64);
+    const long long n4 = (P + 255) / 256;
+    if (n4 * ny >= 512) hipLaunchKernelGGL(k_conv<4>, dim3((unsigned)n4, ny), dim3(256), 0, s, a);
+    else hipLaunchKernelGGL(k_conv<1>, dim3((unsigned)((P + 63) / 64), ny), dim3(256), 0, s, a);
     return hipGetLastError();
 }
 
