@@ -9,6 +9,8 @@ rounding flip that follows from it):
   * every layer's weight gradient within 3 % (relative L2) of the oracle's, cosine > 0.999;
   * one Adam step equal to the oracle's step on the device's own gradient (1e-6 relative);
   * plane-sweep features within 2 bf16 ulps of the oracle's;
+  * a batch large enough for the convolution's four-pixel-group waves gives the same bits as the
+    same images one at a time (one group per wave, the path the oracle comparisons cover);
   * training on plain-PatchStereo labels lowers the loss.
 """
 import numpy as np
@@ -126,6 +128,31 @@ def test_sweep_matches_oracle():
     assert err.max() <= 2 * 2.0 ** -8
     # a pixel near the image border may project just outside in one implementation only
     assert (err > 2 * 2.0 ** -8).mean() == 0
+
+
+def test_forward_large_batch_equals_single_images():
+    """8 images at 272 x 480 run the large layers with four groups of 16 pixels per wave (launches
+    of at least 512 workgroups); one image at a time runs every layer with one group. Each output
+    accumulates in the same order either way, so the head outputs are identical."""
+    rng = np.random.default_rng(21)
+    be = pm.HipPatchMatch()
+    big = init_net.InitNet(be, 8, 272, 480, seed=5)
+    one = init_net.InitNet(be, 1, 272, 480, seed=5)
+    try:
+        one.set_params(big.get_params())
+        feat = rng.uniform(0.0, 1.0, (8, 272, 480, 64)).astype(np.float32)
+        bits = (feat.view(np.uint32) >> 16).astype(np.uint16)
+        big.set_features(bits)
+        _, _, _, hb = big.predict(8, want_head=True)
+        assert np.isfinite(hb).all() and np.abs(hb).max() > 0
+        for i in range(8):
+            one.set_features(bits[i:i + 1])
+            _, _, _, h1 = one.predict(1, want_head=True)
+            assert np.array_equal(h1[0].view(np.uint32), hb[i].view(np.uint32)), i
+    finally:
+        one.close()
+        big.close()
+        be.close()
 
 
 def test_training_lowers_the_loss():
