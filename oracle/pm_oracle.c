@@ -37,7 +37,14 @@
  *                                textures under the true plane give cost 0, constant patch 2,
  *                                projection outside 2
  *   ncc_cost (FP64)              closed forms above and a brute-force double evaluation
- *   pm_half/pm_eval/pm_filter,   synthetic ground truth: depth median rel. error < 0.005,
+ *   pmo_region_pick,             known answers derived by hand from the rule text (kernel.rs:50-66,
+ *   pmo_transfer_plane,          patchmatch.cu:327-345,425-436,714-733, densify.rs:446-454):
+ *   pmo_select_views,            vote edges, previous-view weight, prior-weighted fallback, ties
+ *   pmo_refine_accept,           to the earlier index, acceptance order, perturbation retries,
+ *   pmo_perturb_normal_det,      top-K, read-out tests (tests/test_oracle_rules.py), and planted
+ *   pmo_topk_mean,               slips in them rejected (tests/test_oracle_mutations.py)
+ *   pmo_readout_source
+ *   pm_half/pm_eval/pm_filter,   the rules above, plus synthetic ground truth: depth median rel. error < 0.005,
  *   schedule                     normal median < 3 deg, valid > 0.5 (patchmatch_gpu.rs:96-98
  *                                accuracy bars, applied to this oracle on the floor+boxes scene)
  *   pm_upsample, pm_median       upsample.rs:279-307 (plane reproduced to 1e-5)
