@@ -402,7 +402,11 @@ constexpr int WINDOW_PIPE = 8;
 // four window corners have z > 1e-8: z is monotone in each offset (every step is a correctly
 // rounded affine operation), so then every sample does. With CHECK the sample is computed anyway
 // (any position is clamped into the quad table) and replaced by a select.
-template <int WR, int WSTEP, bool CHECK, int PIPE, class REF>
+// AHEAD: the reference value of a sample is read from LDS with the sample's gather, PIPE samples
+// before it is used, instead of at its use. The half step's windows take it (one fewer wait on
+// LDS per sample; it fits their registers); the evaluation and read-out, held to three waves
+// per SIMD, would spill with it.
+template <int WR, int WSTEP, bool CHECK, int PIPE, bool AHEAD, class REF>
 __device__ __forceinline__ void window_sums(const RefWin<WR, WSTEP>& rw, const REF& ref, int x, int y, const uint32_t* qbase, float qp4,
                                             float koff4, float w1, float h1, const float* H, float& Ss, float& SSs, float& IS) {
     constexpr int WN = Win<WR, WSTEP>::WN;
@@ -413,10 +417,11 @@ __device__ __forceinline__ void window_sums(const RefWin<WR, WSTEP>& rw, const R
     // before sample k is consumed, so PIPE gathers stay in flight. The sums take the samples in
     // row-major order as before.
     constexpr int N = WN * WN;
-    float fx[N], fy[N];
+    float fx[N], fy[N], rv[N];
     uint32_t qv[N];
     auto issue = [&](int k) {
         const int a = k / WN, b = k % WN;
+        if constexpr (AHEAD) rv[k] = ref.at(x + (-WR + b * WSTEP), y + (-WR + a * WSTEP));
         const float oy = (float)(-WR + a * WSTEP);
         const float ox = (float)(-WR + b * WSTEP);
         const float zz = fmaf(H[6], ox, fmaf(H[7], oy, H[8]));
@@ -444,13 +449,13 @@ __device__ __forceinline__ void window_sums(const RefWin<WR, WSTEP>& rw, const R
         float ws = rw.w[k] * sp;
         Ss += ws;
         SSs = fmaf(ws, sp, SSs);
-        IS = fmaf(ws, ref.at(x + (-WR + b * WSTEP), y + (-WR + a * WSTEP)), IS);
+        IS = fmaf(ws, AHEAD ? rv[k] : ref.at(x + (-WR + b * WSTEP), y + (-WR + a * WSTEP)), IS);
     }
 }
 
 // Bilateral-weighted NCC cost 1 - rho, 2 on failure (patchmatch.cu:244-290); the covariance is
 // sum(w s' I)/W - mu mean(s') with s' the shifted source samples.
-template <int WR, int WSTEP, int PIPE = WINDOW_PIPE, class REF>
+template <int WR, int WSTEP, int PIPE = WINDOW_PIPE, bool AHEAD = false, class REF>
 __device__ __forceinline__ float ncc_cost(const KParams& P, const RefWin<WR, WSTEP>& rw, const REF& ref, const LevelDesc& S,
                                           const float* H, int x, int y) {
     float z = H[8];
@@ -468,8 +473,8 @@ __device__ __forceinline__ float ncc_cost(const KParams& P, const RefWin<WR, WST
     const bool all_front = fmaf(H[6], lo, za) > 1e-8f && fmaf(H[6], hi, za) > 1e-8f && fmaf(H[6], lo, zb) > 1e-8f &&
                            fmaf(H[6], hi, zb) > 1e-8f;
     float Ss, SSs, IS;
-    if (all_front) window_sums<WR, WSTEP, false, PIPE>(rw, ref, x, y, qbase, qp4, koff4, w1, h1, H, Ss, SSs, IS);
-    else window_sums<WR, WSTEP, true, PIPE>(rw, ref, x, y, qbase, qp4, koff4, w1, h1, H, Ss, SSs, IS);
+    if (all_front) window_sums<WR, WSTEP, false, PIPE, AHEAD>(rw, ref, x, y, qbase, qp4, koff4, w1, h1, H, Ss, SSs, IS);
+    else window_sums<WR, WSTEP, true, PIPE, AHEAD>(rw, ref, x, y, qbase, qp4, koff4, w1, h1, H, Ss, SSs, IS);
     const float iW = rw.iW;
     float mus = Ss * iW;
     float vars = fmaf(-mus, mus, SSs * iW);
@@ -634,10 +639,10 @@ __device__ __forceinline__ void ref_stats_any(const KParams& P, const REF& ref, 
     else ref_stats_dyn<WSTEP>(P, ref, x, y, wr, rw);
 }
 
-template <int WR, int WSTEP, int PIPE = WINDOW_PIPE, bool CHUNK = false, class REF>
+template <int WR, int WSTEP, int PIPE = WINDOW_PIPE, bool CHUNK = false, bool AHEAD = false, class REF>
 __device__ __forceinline__ float ncc_cost_any(const KParams& P, const RefWinT<WR, WSTEP>& rw, const REF& ref, const LevelDesc& S,
                                               const float* H, int x, int y) {
-    if constexpr (WR > 0) return ncc_cost<WR, WSTEP, PIPE>(P, rw, ref, S, H, x, y);
+    if constexpr (WR > 0) return ncc_cost<WR, WSTEP, PIPE, AHEAD>(P, rw, ref, S, H, x, y);
     else return ncc_cost_dyn<WSTEP, CHUNK>(P, rw, ref, S, H, x, y);
 }
 

@@ -66,7 +66,7 @@ __device__ __forceinline__ TileRefT<WR, TW> tile_ref(const float* tile, int x0, 
 
 // (photometric cost, truncated geometric error) of one hypothesis x one source with the pixel's
 // reciprocal focal lengths hoisted (patchmatch.cu:316-324)
-template <int WR, int WSTEP, bool GEO, int PIPE, class REF>
+template <int WR, int WSTEP, bool GEO, int PIPE, bool AHEAD, class REF>
 __device__ __forceinline__ float pair_cost_h(const KParams& P, const RefWinT<WR, WSTEP>& rw, const REF& ref, const LevelDesc& R,
                                              const LevelDesc& S, const PairRec& pr, const LevelPair& lp, float ifx, float ify, int x,
                                              int y, float4 h, float& geo) {
@@ -75,7 +75,7 @@ __device__ __forceinline__ float pair_cost_h(const KParams& P, const RefWinT<WR,
     // geometric runs: the snapshot depth is fetched before the NCC window and used after it
     GeoFetch gf{0.f, 0.f, 0.f, false};
     if constexpr (GEO) gf = geo_fetch(P, R, S, pr, x, y, h.w);
-    if (homography(R, lp, ifx, ify, (float)x, (float)y, h.w, xyz(h), H)) c = ncc_cost_any<WR, WSTEP, PIPE, GEO>(P, rw, ref, S, H, x, y);
+    if (homography(R, lp, ifx, ify, (float)x, (float)y, h.w, xyz(h), H)) c = ncc_cost_any<WR, WSTEP, PIPE, GEO, AHEAD>(P, rw, ref, S, H, x, y);
     geo = GEO ? fminf(geo_finish(R, S, pr, x, y, gf), P.gmax) : 0.f;
     return c;
 }
@@ -94,6 +94,8 @@ __global__ __launch_bounds__(ROWS * 64) __attribute__((amdgpu_waves_per_eu(2, 2)
     // one lane per pixel: 250 VGPRs), six otherwise, fewer for the 49 register-held weights of a
     // 7 x 7 window (deeper pipelines spill there)
     constexpr int PIPE = Win<(WR > 0 ? WR : 1), WSTEP>::WN2 > 36 ? 3 : ((GEO || TPP > 1) ? 6 : 8);
+    // the reference value of a window sample is read with its gather (see window_sums)
+    constexpr bool AHEAD = Win<(WR > 0 ? WR : 1), WSTEP>::WN2 <= 36;
     constexpr int TW = SPAN + 2 * WR;
     constexpr int TH = ROWS + 2 * WR;
     __shared__ float tile_s[WR > 0 ? TW * TH : 1];
@@ -211,7 +213,7 @@ __global__ __launch_bounds__(ROWS * 64) __attribute__((amdgpu_waves_per_eu(2, 2)
 #pragma nounroll
         for (int i = sub; i < 9; i += TPP) {
             float gi;
-            const float mi = pair_cost_h<WR, WSTEP, GEO, PIPE>(P, rw, ref, R, S, pr, lp, ifx, ify, x, y, hyp[i * NT], gi);
+            const float mi = pair_cost_h<WR, WSTEP, GEO, PIPE, AHEAD>(P, rw, ref, R, S, pr, lp, ifx, ify, x, y, hyp[i * NT], gi);
 #pragma unroll
             for (int q = 0; q < 9; ++q)
                 if (q == i) {
@@ -333,7 +335,7 @@ __global__ __launch_bounds__(ROWS * 64) __attribute__((amdgpu_waves_per_eu(2, 2)
                 const PairRec& pr = prs[j];
                 const LevelDesc S = P.lv[pr.src * P.nlevels + P.level];
                 float g;
-                float m = pair_cost_h<WR, WSTEP, GEO, PIPE>(P, rw, ref, R, S, pr, lps[j], ifx, ify, x, y, hc, g);
+                float m = pair_cost_h<WR, WSTEP, GEO, PIPE, AHEAD>(P, rw, ref, R, S, pr, lps[j], ifx, ify, x, y, hc, g);
                 nm += wj * (m + P.lambda * g);
                 if (early && nm / den >= bcost) {
                     out = true;
