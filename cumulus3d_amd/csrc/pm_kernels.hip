@@ -91,9 +91,9 @@ __global__ __launch_bounds__(ROWS * 64) __attribute__((amdgpu_waves_per_eu(2, 2)
     constexpr int NT = ROWS * 64;
     constexpr int SPAN = HALF_SPAN / TPP;  // pixel span of one wave row (64 / TPP active pixels)
     // source gathers in flight in the window loop: eight where the registers allow (photometric,
-    // one lane per pixel: 250 VGPRs), six otherwise, fewer for the 49 register-held weights of a
-    // 7 x 7 window (deeper pipelines spill there)
-    constexpr int PIPE = Win<(WR > 0 ? WR : 1), WSTEP>::WN2 > 36 ? 3 : ((GEO || TPP > 1) ? 6 : 8);
+    // one lane per pixel: 247 VGPRs), seven in the geometric build (255), six with two lanes per
+    // pixel, fewer for the 49 register-held weights of a 7 x 7 window (deeper pipelines spill)
+    constexpr int PIPE = Win<(WR > 0 ? WR : 1), WSTEP>::WN2 > 36 ? 3 : (TPP > 1 ? 6 : (GEO ? 7 : 8));
     // the reference value of a window sample is read with its gather (see window_sums)
     constexpr bool AHEAD = Win<(WR > 0 ? WR : 1), WSTEP>::WN2 <= 36;
     constexpr int TW = SPAN + 2 * WR;
