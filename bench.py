@@ -66,7 +66,7 @@ HBM_PEAK_GBS = 8000.0
 #          (profiles/k_half-zone7-7x1080p-6src.md)
 #   zone108_1_1x1, (1, 1): the 108-view zone's learned job, 1 refinement + 1x1 geometric
 #          iterations, averaged over its 4 launches (profiles/k_half-zone108-learned.md)
-K_HALF_TRAFFIC = {("zone7", (1, 1)): 42.721e9, ("zone108_1_1x1", (1, 1)): 87.024e9}
+K_HALF_TRAFFIC = {("zone7", (1, 1)): 42.721e9, ("zone108_1_1x1", (1, 1)): 86.917e9}
 FP32_PEAK_TFLOPS = 157.3
 FLOP_PER_SAMPLE = 28  # projective transform 9, reciprocal 1, bilinear 7, scale 1, accumulations 6 (rounded up)
 WN_FAST = 6  # samples per window side at radius 5 step 2
